@@ -1,0 +1,226 @@
+"""Planner benchmark: batched persistent launch vs per-query launches on 1024 uneven 7-DOF queries.
+
+A step is one batched plan of the whole 1024-query workload (one launch of the persistent
+planner).  ``value`` is plans/s of the batched planner with every input already in HBM; the
+same queries are also run as 1024 sequential single-query launches (each followed by a stream
+synchronisation, as a per-query caller would) and both solve rates are reported.  FK+CC
+throughput is measured on 10^6 configurations against the large-scene world.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--queries 1024] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (datasheet)
+PEAK_HBM_GBS = 8000.0
+
+METRIC = "plans/sec + p50 ms/plan, batched vs per-query at equal solve rate; FK+CC cfg/s"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_c3(n_queries: int, checker_factory):
+    """Robot, per-query worlds/goals/starts/max_nodes of the uneven 1024-query workload."""
+    from prrax_amd import scenes
+
+    robot = scenes.make_arm(0, 64, 200)
+    worlds, starts, goals, nodes = [], [], [], []
+    for qi in range(n_queries):
+        w, ng, N = scenes.c3_query_world(robot, qi)
+        cc = dict(robot.ctx)
+        cc.update(w)
+        free = checker_factory(cc)
+        rng = np.random.default_rng(scenes.query_seed(3, qi))
+        sg = scenes.sample_free(rng, robot, 1 + ng, free)
+        worlds.append(cc)
+        starts.append(sg[0])
+        goals.append(sg[1:])
+        nodes.append(N)
+    return robot, worlds, starts, goals, nodes
+
+
+def _cpu_plan(args):
+    cc, start, goals, kw = args
+    from oracle import oracle as O
+
+    r = O.plan(O.Context(cc, 7), start, goals, **kw)
+    return r["status"] == O.STATUS_SOLVED
+
+
+def cpu_baseline(worlds, starts, goals, nodes, kw, n_sample: int, workers: int):
+    jobs = [(worlds[i], starts[i], goals[i], dict(kw, max_nodes=nodes[i])) for i in range(n_sample)]
+    t0 = time.perf_counter()
+    # ctypes releases the GIL for the duration of each oracle call, so threads run in parallel.
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        solved = list(ex.map(_cpu_plan, jobs))
+    dt = time.perf_counter() - t0
+    return {"value": round(n_sample / dt, 3), "unit": "plans/s", "cores": workers, "kind": "port",
+            "sample": f"first {n_sample} of the {len(worlds)} queries, C oracle planner (same lockstep algorithm), "
+                      f"one query per thread",
+            "solve_rate": round(float(np.mean(solved)), 4), "wall_s": round(dt, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--samples", type=int, default=128, help="explorers (samples per round) per query")
+    ap.add_argument("--granularity", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--per-query", type=int, default=-1, help="queries run as single launches (-1 = all)")
+    ap.add_argument("--fkcc-configs", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every query is independent and runs on a single device; "
+              "multi-GPU runs are not part of this build (see DESIGN.md for how the queue would shard).")
+        sys.exit(2)
+
+    import torch
+
+    from prrax_amd import prepare_batch, scenes
+    from prrax_amd.prrtc import CollisionChecker
+
+    torch.cuda.set_device(0)
+    dev_name = torch.cuda.get_device_name(0)
+    log(f"device {dev_name}")
+
+    def checker_factory(cc):
+        chk = CollisionChecker(cc, 7, collision_checker="differentiable", collision_margin=1e-5)
+        return chk.is_free
+
+    t0 = time.perf_counter()
+    robot, worlds, starts, goals, nodes = build_c3(args.queries, checker_factory)
+    log(f"workload built in {time.perf_counter() - t0:.1f}s")
+    kw = dict(max_iterations=10_000, step_size=0.5, num_new_samples=args.samples, granularity=args.granularity,
+              min_vals=robot.lower, max_vals=robot.upper)
+    # Longest queries first (more obstacles -> more work): results do not depend on the order.
+    order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
+                       kind="stable")
+    pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
+                       max_nodes_per_query=nodes, order=order, threads=args.threads, slots=args.slots, **kw)
+    # ---- batched
+    for _ in range(args.warmup):
+        pb.enqueue()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        pb.enqueue()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1000.0 / args.steps
+    summ = pb.bufs.host_summary()
+    stats = pb.bufs.stats()
+    solved_b = summ["solved"] == 1
+    plans_per_s = args.queries / (ms_per_step / 1000.0)
+    log(f"batched: {ms_per_step:.2f} ms/step ({kernel_ms:.2f} ms device), {plans_per_s:.1f} plans/s, "
+        f"solve rate {solved_b.mean():.4f}, {stats}")
+    # ---- per-query launches
+    n_pq = args.queries if args.per_query < 0 else min(args.per_query, args.queries)
+    singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=[worlds[i]], max_nodes=nodes[i],
+                             threads=args.threads, **kw) for i in range(n_pq)]
+    singles[0].enqueue()
+    torch.cuda.synchronize()
+    times = []
+    for sp in singles:
+        t = time.perf_counter()
+        sp.enqueue()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
+    pq_plans_per_s = n_pq / sum(times)
+    same = bool(np.array_equal(solved_pq, solved_b[:n_pq]))
+    log(f"per-query: {pq_plans_per_s:.1f} plans/s, p50 {1000 * np.median(times):.3f} ms, solve rate "
+        f"{solved_pq.mean():.4f}, identical solved flags: {same}")
+    # ---- FK + CC throughput (large-scene world, 10^6 configs)
+    sc2 = scenes.scene_c2()
+    rng = np.random.default_rng(11)
+    qcfg = torch.from_numpy(rng.uniform(sc2.robot.lower, sc2.robot.upper,
+                                        size=(args.fkcc_configs, 7)).astype(np.float32)).cuda()
+    fkcc = {}
+    for mode in ("binary", "differentiable"):
+        chk = CollisionChecker(sc2.context(), 7, collision_checker=mode, collision_margin=0.02)
+        hit = torch.empty(args.fkcc_configs, dtype=torch.uint8, device="cuda")
+        flops = torch.zeros(1, dtype=torch.int64, device="cuda")
+        chk.enqueue(qcfg, hit, None)
+        torch.cuda.synchronize()
+        flops.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 3
+        e0.record()
+        for _ in range(reps):
+            chk.enqueue(qcfg, hit, None, flops)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        fl = float(flops.item()) / reps
+        fkcc[mode] = {"cfg_per_s": round(args.fkcc_configs / (ms / 1000.0), 1), "ms": round(ms, 3),
+                      "tflops": round(fl / (ms / 1000.0) / 1e12, 3),
+                      "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
+        log(f"fkcc {mode}: {fkcc[mode]}")
+    achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(plans_per_s, 2),
+        "unit": "plans/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.queries} uneven 7-DOF queries (10-300 spheres, 0-24 boxes, floor, 1-8 goals, "
+                               "max_nodes 8192-65536), one persistent launch",
+                   "queries": args.queries, "num_new_samples": args.samples, "granularity": args.granularity,
+                   "step_size": 0.5, "max_iterations": 10_000, "threads": args.threads, "slots": args.slots},
+        "batched": {"plans_per_s": round(plans_per_s, 2), "solve_rate": round(float(solved_b.mean()), 4),
+                    "p50_ms_per_plan_device": round(float(np.median(summ["kernel_time_ms"])), 3),
+                    "kernel_ms_per_step": round(kernel_ms, 3), "configs_checked": stats["configs"],
+                    "rounds": stats["rounds"]},
+        "per_query": {"plans_per_s": round(pq_plans_per_s, 2), "p50_ms_per_plan": round(1000 * float(np.median(times)), 3),
+                      "solve_rate": round(float(solved_pq.mean()), 4), "queries": n_pq,
+                      "identical_solved_flags": same},
+        "batched_over_per_query": round(plans_per_s / pq_plans_per_s, 2),
+        "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
+                             "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
+        "roofline": {"kernel": "plan_kernel", "bound": "valu", "achieved": round(achieved, 4),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                     "traffic": None,
+                     "flops_per_launch": stats["flops"]},
+        "device": dev_name,
+    }
+    if not args.no_cpu:
+        workers = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {args.cpu_sample} queries on {workers} threads")
+        out["cpu_baseline"] = cpu_baseline(worlds, starts, goals, nodes, kw, min(args.cpu_sample, args.queries),
+                                           workers)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

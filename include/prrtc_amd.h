@@ -1,0 +1,186 @@
+/*
+ * prrtc_amd.h -- C ABI of the MI355X pRRTC planner library (_prrtc_planner_lib.so).
+ *
+ * Plain pointers and sizes only; every array pointer is DEVICE memory unless stated otherwise,
+ * and every call is enqueue-only on the given HIP stream (no host synchronisation, no device
+ * allocation: scratch comes from the caller's workspace).  Return value: 0 on success, a
+ * negative PRRTC_E_* code otherwise; prrtc_last_error() gives the message (thread-local).
+ *
+ * Each entry point replaces one XLA FFI handler of the reference; the mapping is given on the
+ * declaration.  The reference's argument groups are kept, in order, as structs:
+ *   26 buffer arguments  -> start/goals/min/max + prrtc_robot + prrtc_world (+ coarse arrays,
+ *                           accepted and ignored: the coarse checkers return the flat binary
+ *                           verdict, prrtc_planner.cu:421-425)
+ *   10 results           -> prrtc_outputs (first ten fields, same order and dtypes)
+ *   14 attributes        -> prrtc_attrs (same order and dtypes)
+ */
+#ifndef PRRTC_AMD_H
+#define PRRTC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* prrtc_stream_t; /* hipStream_t */
+
+#define PRRTC_OK 0
+#define PRRTC_E_INVALID -1  /* bad argument (sizes, limits, null pointers) */
+#define PRRTC_E_HIP -2      /* HIP runtime error */
+#define PRRTC_E_WORKSPACE -3/* workspace too small */
+#define PRRTC_E_LDS -4      /* robot + world do not fit the work-group's LDS */
+
+/* Per-query status (the reference only has solved_flag 1/-1, prrtc_planner.cu:776,929,1058). */
+#define PRRTC_STATUS_SOLVED 1
+#define PRRTC_STATUS_ITER_CAP 2
+#define PRRTC_STATUS_TREE_FULL 3
+
+#define PRRTC_MAX_JOINTS 64
+#define PRRTC_MAX_DIM 16
+#define PRRTC_MAX_SAMPLES 1024
+#define PRRTC_MAX_GRANULARITY 64
+
+/* Robot FK + sphere model: collision_context keys fk_* and sphere_* (README.md:81-93),
+ * arguments 5-15 of PrrtcPlannerFfi (prrtc_planner.cu:1075-1085). */
+typedef struct {
+    const float* fk_twists;        /* [n_joints, 6]  (v, omega) */
+    const float* fk_parent_tf;     /* [n_joints, 7]  (w, x, y, z, tx, ty, tz) */
+    const int32_t* fk_parent_idx;  /* [n_joints] */
+    const int32_t* fk_act_idx;     /* [n_joints] */
+    const float* fk_mimic_mul;     /* [n_joints] */
+    const float* fk_mimic_off;     /* [n_joints] */
+    const int32_t* fk_mimic_act_idx; /* [n_joints] */
+    const int32_t* fk_topo_inv;    /* [n_joints] */
+    int32_t n_joints;
+    const int32_t* sphere_link_idx;/* [n_robot_spheres] */
+    const float* sphere_local;     /* [n_robot_spheres, 3] */
+    const float* sphere_radius;    /* [n_robot_spheres] */
+    int32_t n_robot_spheres;
+} prrtc_robot;
+
+/* World primitives + self pairs: arguments 16-20 of PrrtcPlannerFfi (prrtc_planner.cu:1086-1090).
+ * Single-query calls: arrays hold n_* rows.  Batch calls with per-query worlds
+ * (PrrtcPlannerBatchCtxFfi, prrtc_planner.cu:1946-2005): arrays are padded [batch, n_*, F] and
+ * counts is [batch, 5] (spheres, capsules, boxes, halfspaces, self pairs) -- the reference passes
+ * these as five [batch] vectors; here they are interleaved per query. */
+typedef struct {
+    const float* world_spheres;    /* [.., 4]  (x, y, z, r) */
+    const float* world_capsules;   /* [.., 7]  (p1, p2, r) */
+    const float* world_boxes;      /* [.., 15] (c, a1, a2, a3, half lengths) */
+    const float* world_halfspaces; /* [.., 6]  (n, p); free side is +n */
+    const int32_t* self_pairs;     /* [.., 2] */
+    int32_t n_world_spheres, n_world_capsules, n_world_boxes, n_world_halfspaces, n_self_pairs;
+    const int32_t* counts;         /* batch only: [batch, 5] or NULL (every query uses n_*) */
+} prrtc_world;
+
+/* The 14 FFI attributes, same order (prrtc_planner.cu:2103-2116, prrtc.py:537-550). */
+typedef struct {
+    int32_t max_iterations;
+    float step_size;
+    int32_t num_new_samples;
+    int32_t balance_mode;
+    float tree_ratio;
+    int32_t dynamic_domain;
+    float dd_alpha;
+    float dd_radius;
+    float dd_min_radius;
+    int32_t dim;
+    int32_t max_nodes;
+    int32_t granularity;
+    int32_t collision_mode;  /* 0 binary, 1 margin ("differentiable"), 2/3 coarse (binary verdict) */
+    float collision_margin;
+} prrtc_attrs;
+
+/* The 10 FFI results (prrtc.py:384-395), plus optional extras (NULL = not wanted).
+ * Batch calls prefix every array with [batch] (prrtc.py:1088-1099). */
+typedef struct {
+    float* tree_a_configs;     /* [dim, max_nodes] SoA */
+    float* tree_b_configs;     /* [dim, max_nodes] */
+    int32_t* tree_a_parents;   /* [max_nodes]; a root is its own parent */
+    int32_t* tree_b_parents;   /* [max_nodes] */
+    int32_t* tree_sizes;       /* [2] */
+    int32_t* completed;        /* [2] (equal to tree_sizes: every counted node is written) */
+    int32_t* iter_count;       /* [1] lockstep rounds run */
+    int32_t* connection_info;  /* [3] a_idx, b_idx, extended tree */
+    int32_t* solved_flag;      /* [1] 1 solved, -1 failed */
+    float* kernel_time_ms;     /* [1] device time of this query */
+    int32_t* status;           /* [1] PRRTC_STATUS_* (optional) */
+    float* path;               /* [path_capacity, dim] start -> goal (optional) */
+    int32_t* path_len;         /* [1] nodes written, 0 if unsolved, -1 if over capacity */
+    int32_t path_capacity;
+} prrtc_outputs;
+
+/* Launch shape (NULL = defaults chosen by the library). */
+typedef struct {
+    int32_t threads;  /* threads per work-group, multiple of 64, <= 1024; 0 = auto */
+    int32_t slots;    /* resident work-groups (queries in flight); 0 = auto */
+} prrtc_launch;
+
+/* Workspace bytes a planner call needs (radii + per-query explorer state + queue counters). */
+size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
+
+/* Byte offset, inside that workspace, of four uint64 counters the last planner call left there:
+ * [0] configurations FK+collision-checked, [1] lockstep rounds summed over queries,
+ * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] reserved.  No reference
+ * counterpart (the reference reports only kernel_time_ms). */
+size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
+
+/* Single query.  Replaces PrrtcPlannerFfi (prrtc_planner.cu:2063-2117, handler 1069-1396):
+ * start [dim], goals [num_goals, dim], min/max [dim]. */
+int prrtc_planner(prrtc_stream_t stream, const float* start, const float* goals, int32_t num_goals,
+                  const float* min_vals, const float* max_vals, const prrtc_robot* robot, const prrtc_world* world,
+                  const prrtc_attrs* attrs, const prrtc_outputs* out, const prrtc_launch* launch, void* workspace,
+                  size_t workspace_bytes);
+
+/* Batch of independent queries in one persistent launch.  Replaces PrrtcPlannerBatchFfi
+ * (prrtc_planner.cu:2007-2061; shared world: world->counts == NULL and world_per_query == 0) and
+ * PrrtcPlannerBatchCtxFfi (prrtc_planner.cu:1946-2005; padded per-query worlds, world_per_query=1).
+ * starts [batch, dim], goals [batch, max_goals, dim].  Extensions over the reference (all may be
+ * NULL): goal_counts [batch] (uneven goal sets), max_nodes_q [batch] (per-query tree capacity,
+ * <= attrs->max_nodes), seeds [batch] (sampler seed; 0 = the reference's sequence), order [batch]
+ * (queue order, a permutation of 0..batch-1; results do not depend on it). */
+int prrtc_planner_batch(prrtc_stream_t stream, int32_t batch, const float* starts, const float* goals,
+                        int32_t max_goals, const int32_t* goal_counts, const float* min_vals, const float* max_vals,
+                        const prrtc_robot* robot, const prrtc_world* world, int32_t world_per_query,
+                        const int32_t* max_nodes_q, const uint32_t* seeds, const int32_t* order,
+                        const prrtc_attrs* attrs, const prrtc_outputs* out, const prrtc_launch* launch,
+                        void* workspace, size_t workspace_bytes);
+
+/* Collision verdict of n configs [n, dim] against one robot + world (no reference FFI target;
+ * it is the planner's prrtc_config_in_collision, prrtc_planner.cu:501-531, exposed for parity
+ * and throughput).  out_hit [n] uint8 (may be NULL), out_min_dist [n] float (may be NULL; only
+ * computed in collision_mode 1 or when out_hit is NULL).  work_flops (may be NULL): a device
+ * uint64 that the call atomically increments by the algorithmic flops of the checking it did. */
+int prrtc_check_configs(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim, const prrtc_robot* robot,
+                        const prrtc_world* world, int32_t collision_mode, float collision_margin, uint8_t* out_hit,
+                        float* out_min_dist, const prrtc_launch* launch, uint64_t* work_flops);
+
+/* FK of n configs: out [n, n_joints, 7] world poses in the original joint order (pyroffi
+ * fk_single semantics, prrtc_planner.cu:509-522). */
+int prrtc_forward_kinematics(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim,
+                             const prrtc_robot* robot, float* out);
+
+/* Replaces PrrtcNearestNeighborFfi (prrtc_nearest_neighbor.cu:86-141): nearest node of each
+ * query [batch, dim] among the first tree_size nodes of tree [dim, max_nodes]; ties go to the
+ * lowest index; dists are Euclidean (sqrt), as the reference returns. */
+int prrtc_nearest_neighbor(prrtc_stream_t stream, const float* tree_configs, int32_t dim, int32_t max_nodes,
+                           int32_t tree_size, const float* queries, int32_t batch, int32_t* out_idx, float* out_dist);
+
+/* Replaces PrrtcExtendFfi (prrtc_extend.cu:84-140): steer from tree node nearest[b] toward
+ * samples[b] by at most step_size (prrtc_planner.cu:870-876 arithmetic, with its 1e-8 guard). */
+int prrtc_extend(prrtc_stream_t stream, const float* tree_configs, int32_t dim, int32_t max_nodes,
+                 const int32_t* nearest, const float* samples, int32_t batch, float step_size, float* out_configs,
+                 int32_t* out_parents, int32_t* out_valid);
+
+/* Message of the last failing call on this thread. */
+const char* prrtc_last_error(void);
+
+/* Library version string. */
+const char* prrtc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

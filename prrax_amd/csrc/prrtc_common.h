@@ -1,0 +1,293 @@
+// prrtc_common.h -- LDS staging of robot + world and work-group primitives shared by the
+// planner and the standalone kernels.
+#pragma once
+#include "prrtc_device.h"
+
+namespace prrtc {
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
+
+// Device-side views of the C-ABI robot and world (by value in kernel arguments).
+struct RobotDev {
+    const float* twists;
+    const float* parent_tf;
+    const int* parent_idx;
+    const int* act_idx;
+    const float* mimic_mul;
+    const float* mimic_off;
+    const int* mimic_act_idx;
+    const int* topo_inv;
+    int n_joints;
+    const int* sphere_link_idx;
+    const float* sphere_local;
+    const float* sphere_radius;
+    int n_spheres;
+};
+
+struct WorldDev {
+    const float* ws;
+    const float* wc;
+    const float* wb;
+    const float* wh;
+    const int* sp;
+    int n_ws, n_wc, n_wb, n_wh, n_sp;              // capacity (padded) counts
+    long long s_ws, s_wc, s_wb, s_wh, s_sp;        // per-query strides in elements (0 = shared)
+    const int* counts;                             // [B,5] or null
+};
+
+// Byte offsets of the model inside dynamic LDS, followed by per-lane scratch.
+struct ModelLayout {
+    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, lane_q, lane_T, end;
+};
+
+__host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, int nws, int nwc, int nwb, int nwh,
+                                                      int nt) {
+    ModelLayout L;
+    size_t o = 0;
+    L.joints = o; o = align16(o + sizeof(JointRec) * nj);
+    L.spheres = o; o = align16(o + sizeof(float4) * nrs);
+    L.sphere_js = o; o = align16(o + sizeof(int) * nrs);
+    L.pairs = o; o = align16(o + sizeof(int2) * nsp);
+    L.wsph = o; o = align16(o + sizeof(float4) * nws);
+    L.wcap = o; o = align16(o + sizeof(CapsuleL) * nwc);
+    L.wbox = o; o = align16(o + sizeof(BoxL) * nwb);
+    L.whs = o; o = align16(o + sizeof(float4) * nwh);
+    L.lane_q = o; o = align16(o + sizeof(float) * kMaxDim * nt);
+    L.lane_T = o; o = align16(o + sizeof(float) * 7 * nj * nt);
+    L.end = o;
+    return L;
+}
+
+// ---------------------------------------------------------------------------------- LDS atomics
+// Atomics on LDS words through explicit address-space-3 pointers, so they lower to ds_* atomics
+// (no generic-pointer address-space test).
+#define PRRTC_LDS __attribute__((address_space(3)))
+__device__ __forceinline__ PRRTC_LDS int* as_lds(int* p) { return (PRRTC_LDS int*)(p); }
+// Flag reads/writes are volatile word accesses (LDS is coherent within the work-group); atomic
+// load/store builtins on these pointers hit a backend legalisation error when packed FP32 is
+// disabled.
+__device__ __forceinline__ int lds_load(int* p) { return *reinterpret_cast<volatile int*>(p); }
+__device__ __forceinline__ void lds_store(int* p, int v) { *reinterpret_cast<volatile int*>(p) = v; }
+__device__ __forceinline__ void lds_min(int* p, int v) {
+    __hip_atomic_fetch_min(as_lds(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(int* p, int v) {
+    __hip_atomic_fetch_add(as_lds(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int lds_cas(int* p, int expected, int desired) {
+    __hip_atomic_compare_exchange_strong(as_lds(p), &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return expected;
+}
+
+// Device-scope atomics on global memory through address-space-1 pointers (plain global_atomic_*;
+// a generic pointer would make the compiler test the address space at run time).
+#define PRRTC_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ int global_add(int* p, int v) {
+    return __hip_atomic_fetch_add((PRRTC_GLOBAL int*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void global_add(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned long long*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------------- scans
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix sum over the work-group; scratch holds >= 17 ints of LDS.  Every thread must
+// call it.  Returns the exclusive prefix of v; *total receives the sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* scratch, int* total) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wid = tid / kWave, nw = (blockDim.x + kWave - 1) / kWave;
+    const int incl = wave_incl_scan(v);
+    if (lane == kWave - 1 || tid == static_cast<int>(blockDim.x) - 1) scratch[wid] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int t = scratch[w];
+            scratch[w] = run;
+            run += t;
+        }
+        scratch[16] = run;
+    }
+    __syncthreads();
+    const int res = scratch[wid] + incl - v;
+    *total = scratch[16];
+    __syncthreads();
+    return res;
+}
+
+// ---------------------------------------------------------------------------------- staging
+// Stages robot + world of query q into LDS and fills M.  Must be called by the whole
+// work-group; ends with a barrier.  pair_cap is the LDS capacity for self pairs.
+// Indices that would reach outside the staged arrays (a parent that is not an earlier joint, an
+// actuated/mimic index outside [0, dim), a topo_inv entry outside [0, n_joints)) are treated as
+// absent; the host-side context check rejects such robots before they get here.
+__device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, const RobotDev& r, const WorldDev& w,
+                                   int q, int dim, int mode, float margin, int* scan_scratch) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    JointRec* J = reinterpret_cast<JointRec*>(lds + Lo.joints);
+    float4* SP = reinterpret_cast<float4*>(lds + Lo.spheres);
+    int* SJ = reinterpret_cast<int*>(lds + Lo.sphere_js);
+    int2* PR = reinterpret_cast<int2*>(lds + Lo.pairs);
+    float4* WS = reinterpret_cast<float4*>(lds + Lo.wsph);
+    CapsuleL* WC = reinterpret_cast<CapsuleL*>(lds + Lo.wcap);
+    BoxL* WB = reinterpret_cast<BoxL*>(lds + Lo.wbox);
+    float4* WH = reinterpret_cast<float4*>(lds + Lo.whs);
+    const int nj = r.n_joints, nrs = r.n_spheres;
+    int nws = w.n_ws, nwc = w.n_wc, nwb = w.n_wb, nwh = w.n_wh, nsp = w.n_sp;
+    if (w.counts) {
+        const int* c = w.counts + 5 * q;
+        nws = min(c[0], w.n_ws); nwc = min(c[1], w.n_wc); nwb = min(c[2], w.n_wb);
+        nwh = min(c[3], w.n_wh); nsp = min(c[4], w.n_sp);
+    }
+    const float* ws = w.ws + w.s_ws * q;
+    const float* wc = w.wc + w.s_wc * q;
+    const float* wb = w.wb + w.s_wb * q;
+    const float* wh = w.wh + w.s_wh * q;
+    const int* sp = w.sp + w.s_sp * q;
+    for (int j = tid; j < nj; j += nt) {
+        JointRec rec;
+        joint_prep(r.twists + 6 * j, rec);
+        const int p = r.parent_idx[j], ac = r.act_idx[j], mi = r.mimic_act_idx[j];
+        rec.parent = (p >= 0 && p < j) ? p : -1;
+        rec.act = (ac >= 0 && ac < dim) ? ac : -1;
+        rec.mimic = (mi >= 0 && mi < dim) ? mi : -1;
+        rec.mul = r.mimic_mul[j];
+        rec.off = r.mimic_off[j];
+        const float* P = r.parent_tf + 7 * j;
+        rec.pw = P[0]; rec.px = P[1]; rec.py = P[2]; rec.pz = P[3];
+        rec.ptx = P[4]; rec.pty = P[5]; rec.ptz = P[6];
+        J[j] = rec;
+    }
+    for (int s = tid; s < nrs; s += nt) {
+        const int l = r.sphere_link_idx[s];
+        const int ti = (l >= 0 && l < nj) ? r.topo_inv[l] : -1;
+        SJ[s] = (ti >= 0 && ti < nj) ? ti : -1;
+        SP[s] = make_float4(r.sphere_local[3 * s], r.sphere_local[3 * s + 1], r.sphere_local[3 * s + 2],
+                            r.sphere_radius[s]);
+    }
+    for (int k = tid; k < nws; k += nt)
+        WS[k] = make_float4(ws[4 * k], ws[4 * k + 1], ws[4 * k + 2], ws[4 * k + 3]);
+    for (int k = tid; k < nwc; k += nt) WC[k] = make_capsule(wc + 7 * k);
+    for (int k = tid; k < nwb; k += nt) {
+        const float* b = wb + 15 * k;
+        BoxL o;
+        o.cx = b[0]; o.cy = b[1]; o.cz = b[2];
+        o.a0x = b[3]; o.a0y = b[4]; o.a0z = b[5];
+        o.a1x = b[6]; o.a1y = b[7]; o.a1z = b[8];
+        o.a2x = b[9]; o.a2y = b[10]; o.a2z = b[11];
+        o.h0 = b[12]; o.h1 = b[13]; o.h2 = b[14]; o.pad = 0.0f;
+        WB[k] = o;
+    }
+    for (int k = tid; k < nwh; k += nt) WH[k] = make_halfspace(wh + 6 * k);
+    __syncthreads();
+    // Self pairs: keep the valid ones (both indices in range, both spheres on a valid frame),
+    // in their original order.
+    int base = 0;
+    for (int k0 = 0; k0 < nsp; k0 += nt) {
+        const int k = k0 + tid;
+        int i = -1, j = -1, ok = 0;
+        if (k < nsp) {
+            i = sp[2 * k]; j = sp[2 * k + 1];
+            ok = (i >= 0 && j >= 0 && i < nrs && j < nrs && SJ[i] >= 0 && SJ[j] >= 0) ? 1 : 0;
+        }
+        int tot;
+        const int pre = block_excl_scan(ok, scan_scratch, &tot);
+        if (ok) PR[base + pre] = make_int2(i, j);
+        base += tot;
+    }
+    M.joints = J; M.spheres = SP; M.sphere_js = SJ; M.pairs = PR;
+    M.wsph = WS; M.wcap = WC; M.wbox = WB; M.whs = WH;
+    M.n_joints = nj; M.n_spheres = nrs; M.n_pairs = base;
+    M.n_ws = nws; M.n_wc = nwc; M.n_wb = nwb; M.n_wh = nwh;
+    M.enabled = (nj > 0 && nrs > 0 && (nws > 0 || nwc > 0 || nwb > 0 || nwh > 0 || nsp > 0)) ? 1 : 0;
+    M.mode = mode;
+    M.margin = margin;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------- nearest neighbour
+constexpr int kEC = 16;  // queries per tree pass
+
+__host__ __device__ inline size_t nn_bytes() { return align16(sizeof(float) * kEC * kMaxDim + 2 * sizeof(int) * kEC * 16); }
+
+__device__ __forceinline__ bool lex_less(float da, int ia, float db, int ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// Nearest node (lowest index on ties) of each listed explorer's query among tree[0, size).
+template <int MAXD>
+__device__ void nn_tiles(const float* __restrict__ tree, int ncap, int size, int dim, const int* list, int nl,
+                         const float* __restrict__ qbuf, int* out_i, float* out_d, char* nnl) {
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int nw = (nt + kWave - 1) / kWave;
+    float* qsh = reinterpret_cast<float*>(nnl);
+    float* red_d = qsh + kEC * kMaxDim;
+    int* red_i = reinterpret_cast<int*>(red_d + kEC * 16);
+    for (int c0 = 0; c0 < nl; c0 += kEC) {
+        const int nc = min(kEC, nl - c0);
+        for (int k = tid; k < kEC * MAXD; k += nt) {
+            const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
+            qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qbuf[list[c0 + c] * dim + d] : 0.0f;
+        }
+        __syncthreads();
+        float bd[kEC];
+        int bi[kEC];
+#pragma unroll
+        for (int c = 0; c < kEC; ++c) { bd[c] = FLT_MAX; bi[c] = 0; }
+        for (int i = tid; i < size; i += nt) {
+            float x[MAXD];
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? tree[static_cast<size_t>(d) * ncap + i] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < kEC; ++c) {
+                float dsq = 0.0f;
+#pragma unroll
+                for (int d = 0; d < MAXD; ++d) {
+                    if (d < dim) {
+                        const float df = x[d] - qsh[c * kMaxDim + d];
+                        dsq = __builtin_fmaf(df, df, dsq);
+                    }
+                }
+                if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kEC; ++c) {
+            float d = bd[c];
+            int i = bi[c];
+#pragma unroll
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const float od = __shfl_xor(d, o, kWave);
+                const int oi = __shfl_xor(i, o, kWave);
+                if (lex_less(od, oi, d, i)) { d = od; i = oi; }
+            }
+            if (lane == 0) { red_d[c * 16 + wid] = d; red_i[c * 16 + wid] = i; }
+        }
+        __syncthreads();
+        if (tid < nc) {
+            float d = red_d[tid * 16];
+            int i = red_i[tid * 16];
+            for (int w = 1; w < nw; ++w) {
+                const float od = red_d[tid * 16 + w];
+                const int oi = red_i[tid * 16 + w];
+                if (lex_less(od, oi, d, i)) { d = od; i = oi; }
+            }
+            const int e = list[c0 + tid];
+            out_i[e] = i;
+            out_d[e] = d;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace prrtc

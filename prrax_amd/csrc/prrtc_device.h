@@ -1,0 +1,431 @@
+// prrtc_device.h -- device-side arithmetic for the MI355X pRRTC planner (gfx950, wave64).
+//
+// Forward kinematics (pyroffi fk_single, called at cuda-rrtc/CUDA/prrtc_planner.cu:509-522),
+// the SE3 point transform (apply_se3_point, prrtc_planner.cu:246) and the primitive collision
+// tests (sphere_*_dist, prrtc_planner.cu:250-304) are written here from the mathematics.
+//
+// Floating-point contract: this file is compiled with -ffp-contract=off and every fused
+// multiply-add is an explicit __builtin_fmaf, so each value is one fixed sequence of IEEE
+// operations.  sin/cos come from the polynomial below rather than the device library.  The
+// CPU oracle restates the same sequence independently; planner trees are compared bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+namespace prrtc {
+
+constexpr int kMaxJoints = 64;
+constexpr int kMaxDim = 16;
+constexpr int kWave = 64;
+
+enum JointType : int { kFixed = 0, kRot = 1, kPrism = 2, kScrew = 3 };
+
+// ---------------------------------------------------------------------------------- sin/cos
+// Cody-Waite reduction by pi/2 (three-part split) and Taylor polynomials on [-pi/4, pi/4];
+// coefficients are (-1)^k/n! rounded to float.
+__device__ __forceinline__ void sincos_pr(float x, float& s_out, float& c_out) {
+    const float k = __builtin_rintf(x * 0x1.45f306p-1f);
+    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
+    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps = -0x1.ae6456p-26f;
+    ps = __builtin_fmaf(ps, r2, 0x1.71de3ap-19f);
+    ps = __builtin_fmaf(ps, r2, -0x1.a01a02p-13f);
+    ps = __builtin_fmaf(ps, r2, 0x1.111112p-7f);
+    ps = __builtin_fmaf(ps, r2, -0x1.555556p-3f);
+    const float sn = __builtin_fmaf(r * r2, ps, r);
+    float pc = 0x1.1eed8ep-29f;
+    pc = __builtin_fmaf(pc, r2, -0x1.27e4fcp-22f);
+    pc = __builtin_fmaf(pc, r2, 0x1.a01a02p-16f);
+    pc = __builtin_fmaf(pc, r2, -0x1.6c16c2p-10f);
+    pc = __builtin_fmaf(pc, r2, 0x1.555556p-5f);
+    const float cs = __builtin_fmaf(r2, __builtin_fmaf(r2, pc, -0.5f), 1.0f);
+    const int q = static_cast<int>(k) & 3;
+    const float a = (q & 1) ? cs : sn;
+    const float b = (q & 1) ? sn : cs;
+    s_out = (q & 2) ? -a : a;
+    c_out = ((q + 1) & 2) ? -b : b;
+}
+
+// ---------------------------------------------------------------------------------- SE3
+struct Quat { float w, x, y, z; };
+struct Mat3 { float m0, m1, m2, m3, m4, m5, m6, m7, m8; };
+
+__device__ __forceinline__ Quat qmul(const Quat& a, const Quat& b) {
+    Quat o;
+    float t;
+    t = a.w * b.w; t = __builtin_fmaf(-a.x, b.x, t); t = __builtin_fmaf(-a.y, b.y, t); t = __builtin_fmaf(-a.z, b.z, t); o.w = t;
+    t = a.w * b.x; t = __builtin_fmaf(a.x, b.w, t); t = __builtin_fmaf(a.y, b.z, t); t = __builtin_fmaf(-a.z, b.y, t); o.x = t;
+    t = a.w * b.y; t = __builtin_fmaf(-a.x, b.z, t); t = __builtin_fmaf(a.y, b.w, t); t = __builtin_fmaf(a.z, b.x, t); o.y = t;
+    t = a.w * b.z; t = __builtin_fmaf(a.x, b.y, t); t = __builtin_fmaf(-a.y, b.x, t); t = __builtin_fmaf(a.z, b.w, t); o.z = t;
+    return o;
+}
+
+__device__ __forceinline__ Mat3 quat_mat(const Quat& q) {
+    const float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+    const float xx = q.x * x2, yy = q.y * y2, zz = q.z * z2;
+    const float xy = q.x * y2, xz = q.x * z2, yz = q.y * z2;
+    const float wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
+    Mat3 R;
+    R.m0 = 1.0f - (yy + zz); R.m1 = xy - wz;          R.m2 = xz + wy;
+    R.m3 = xy + wz;          R.m4 = 1.0f - (xx + zz); R.m5 = yz - wx;
+    R.m6 = xz - wy;          R.m7 = yz + wx;          R.m8 = 1.0f - (xx + yy);
+    return R;
+}
+
+__device__ __forceinline__ float3 mat_apply(const Mat3& R, float tx, float ty, float tz, float px, float py, float pz) {
+    float3 o;
+    o.x = __builtin_fmaf(R.m2, pz, __builtin_fmaf(R.m1, py, __builtin_fmaf(R.m0, px, tx)));
+    o.y = __builtin_fmaf(R.m5, pz, __builtin_fmaf(R.m4, py, __builtin_fmaf(R.m3, px, ty)));
+    o.z = __builtin_fmaf(R.m8, pz, __builtin_fmaf(R.m7, py, __builtin_fmaf(R.m6, px, tz)));
+    return o;
+}
+
+// ---------------------------------------------------------------------------------- model
+// Joint record staged in LDS once per query (topological order of the context arrays).
+struct JointRec {
+    int type, parent, act, mimic;
+    float mul, off, wn, ax, ay, az, vx, vy, vz;
+    float pw, px, py, pz, ptx, pty, ptz;
+};
+
+// Classify a twist (v, omega) and derive its unit axis; same decisions as the oracle.
+__device__ __forceinline__ void joint_prep(const float* tw, JointRec& j) {
+    const float vx = tw[0], vy = tw[1], vz = tw[2], wx = tw[3], wy = tw[4], wz = tw[5];
+    const float wn2 = __builtin_fmaf(wz, wz, __builtin_fmaf(wy, wy, wx * wx));
+    const float vn2 = __builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx));
+    j.vx = vx; j.vy = vy; j.vz = vz;
+    j.ax = 0.0f; j.ay = 0.0f; j.az = 0.0f; j.wn = 0.0f;
+    if (wn2 == 0.0f && vn2 == 0.0f) {
+        j.type = kFixed;
+    } else if (wn2 == 0.0f) {
+        j.type = kPrism;
+    } else {
+        const float wn = __builtin_sqrtf(wn2);
+        j.wn = wn;
+        j.ax = wx / wn; j.ay = wy / wn; j.az = wz / wn;
+        j.type = (vn2 == 0.0f) ? kRot : kScrew;
+    }
+}
+
+// World primitives in LDS, in the form the tests consume (derived values use the same
+// operation sequence as the oracle's inline computation).
+struct CapsuleL { float p1x, p1y, p1z, ax, ay, az, inv, r; };
+struct BoxL { float cx, cy, cz, a0x, a0y, a0z, a1x, a1y, a1z, a2x, a2y, a2z, h0, h1, h2, pad; };
+
+__device__ __forceinline__ CapsuleL make_capsule(const float* c) {
+    CapsuleL o;
+    o.p1x = c[0]; o.p1y = c[1]; o.p1z = c[2];
+    o.ax = c[3] - c[0]; o.ay = c[4] - c[1]; o.az = c[5] - c[2];
+    const float l2 = __builtin_fmaf(o.az, o.az, __builtin_fmaf(o.ay, o.ay, o.ax * o.ax));
+    o.inv = (l2 > 0.0f) ? 1.0f / l2 : 0.0f;
+    o.r = c[6];
+    return o;
+}
+
+__device__ __forceinline__ float4 make_halfspace(const float* h) {
+    const float off = __builtin_fmaf(h[2], h[5], __builtin_fmaf(h[1], h[4], h[0] * h[3]));
+    return make_float4(h[0], h[1], h[2], off);
+}
+
+// Robot + world as seen by one work-group (all pointers into LDS).
+struct Model {
+    const JointRec* joints;
+    const float4* spheres;   // robot spheres: local xyz, radius
+    const int* sphere_js;    // sorted-joint index of each robot sphere, -1 if skipped
+    const int2* pairs;       // valid self pairs
+    const float4* wsph;
+    const CapsuleL* wcap;
+    const BoxL* wbox;
+    const float4* whs;
+    int n_joints, n_spheres, n_pairs, n_ws, n_wc, n_wb, n_wh;
+    int enabled;
+    int mode;       // 0 binary (also the coarse modes: same verdict), 1 margin
+    float margin;
+};
+
+// ---------------------------------------------------------------------------------- tests
+__device__ __forceinline__ bool hit_sphere(float sx, float sy, float sz, float sr, const float4 o) {
+    const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float rr = sr + o.w;
+    return d2 <= rr * rr;
+}
+
+__device__ __forceinline__ float capsule_d2(float sx, float sy, float sz, const CapsuleL& c) {
+    const float wx = sx - c.p1x, wy = sy - c.p1y, wz = sz - c.p1z;
+    float t = __builtin_fmaf(wz, c.az, __builtin_fmaf(wy, c.ay, wx * c.ax)) * c.inv;
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    const float dx = __builtin_fmaf(-t, c.ax, wx), dy = __builtin_fmaf(-t, c.ay, wy), dz = __builtin_fmaf(-t, c.az, wz);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ bool hit_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c) {
+    const float rr = sr + c.r;
+    return capsule_d2(sx, sy, sz, c) <= rr * rr;
+}
+
+__device__ __forceinline__ void box_q(float sx, float sy, float sz, const BoxL& b, float& q0, float& q1, float& q2) {
+    const float wx = sx - b.cx, wy = sy - b.cy, wz = sz - b.cz;
+    const float l0 = __builtin_fmaf(wz, b.a0z, __builtin_fmaf(wy, b.a0y, wx * b.a0x));
+    const float l1 = __builtin_fmaf(wz, b.a1z, __builtin_fmaf(wy, b.a1y, wx * b.a1x));
+    const float l2 = __builtin_fmaf(wz, b.a2z, __builtin_fmaf(wy, b.a2y, wx * b.a2x));
+    q0 = fabsf(l0) - b.h0;
+    q1 = fabsf(l1) - b.h1;
+    q2 = fabsf(l2) - b.h2;
+}
+
+__device__ __forceinline__ bool hit_box(float sx, float sy, float sz, float sr, const BoxL& b) {
+    float q0, q1, q2;
+    box_q(sx, sy, sz, b, q0, q1, q2);
+    const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
+    const float d2 = __builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0));
+    return d2 <= sr * sr;
+}
+
+__device__ __forceinline__ bool hit_halfspace(float sx, float sy, float sz, float sr, const float4 h) {
+    const float s = __builtin_fmaf(h.z, sz, __builtin_fmaf(h.y, sy, h.x * sx)) - h.w;
+    return s <= sr;
+}
+
+__device__ __forceinline__ float dist_sphere(float sx, float sy, float sz, float sr, const float4 o) {
+    const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
+    return __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) - (sr + o.w);
+}
+
+__device__ __forceinline__ float dist_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c) {
+    return __builtin_sqrtf(capsule_d2(sx, sy, sz, c)) - (sr + c.r);
+}
+
+__device__ __forceinline__ float dist_box(float sx, float sy, float sz, float sr, const BoxL& b) {
+    float q0, q1, q2;
+    box_q(sx, sy, sz, b, q0, q1, q2);
+    const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
+    const float outside = __builtin_sqrtf(__builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0)));
+    const float inside = fminf(fmaxf(q0, fmaxf(q1, q2)), 0.0f);
+    return (outside + inside) - sr;
+}
+
+__device__ __forceinline__ float dist_halfspace(float sx, float sy, float sz, float sr, const float4 h) {
+    return (__builtin_fmaf(h.z, sz, __builtin_fmaf(h.y, sy, h.x * sx)) - h.w) - sr;
+}
+
+// ---------------------------------------------------------------------------------- FK
+// Per-lane scratch lives in LDS in lane-minor layout: element c of lane l at [c * nt + l], so
+// a wave's 64 lanes touch 64 consecutive words (no bank conflicts).  q holds the config
+// (kMaxDim slots), T holds 7 floats per joint (w, x, y, z, tx, ty, tz).
+struct LaneScratch {
+    float* q;
+    float* T;
+    int nt;
+};
+
+__device__ __forceinline__ void fk_lane(const Model& M, const LaneScratch& L) {
+    const int nt = L.nt;
+    Quat pq = {1.0f, 0.0f, 0.0f, 0.0f};
+    float ptx = 0.0f, pty = 0.0f, ptz = 0.0f;
+    for (int j = 0; j < M.n_joints; ++j) {
+        const JointRec& J = M.joints[j];
+        float theta;
+        if (J.mimic >= 0) theta = __builtin_fmaf(L.q[J.mimic * nt], J.mul, J.off);
+        else if (J.act >= 0) theta = L.q[J.act * nt];
+        else theta = 0.0f;
+        Quat aq;
+        float atx, aty, atz;
+        const Quat P = {J.pw, J.px, J.py, J.pz};
+        if (J.parent < 0) {
+            aq = P; atx = J.ptx; aty = J.pty; atz = J.ptz;
+        } else {
+            Quat tq;
+            float ttx, tty, ttz;
+            if (J.parent == j - 1) {
+                tq = pq; ttx = ptx; tty = pty; ttz = ptz;
+            } else {
+                const float* Tp = L.T + (J.parent * 7) * nt;
+                tq.w = Tp[0]; tq.x = Tp[nt]; tq.y = Tp[2 * nt]; tq.z = Tp[3 * nt];
+                ttx = Tp[4 * nt]; tty = Tp[5 * nt]; ttz = Tp[6 * nt];
+            }
+            aq = qmul(tq, P);
+            const float3 t = mat_apply(quat_mat(tq), ttx, tty, ttz, J.ptx, J.pty, J.ptz);
+            atx = t.x; aty = t.y; atz = t.z;
+        }
+        Quat oq = aq;
+        float otx = atx, oty = aty, otz = atz;
+        if (J.type == kRot) {
+            float s, c;
+            sincos_pr(0.5f * (J.wn * theta), s, c);
+            const Quat E = {c, s * J.ax, s * J.ay, s * J.az};
+            oq = qmul(aq, E);
+        } else if (J.type == kPrism) {
+            const float3 t = mat_apply(quat_mat(aq), atx, aty, atz, J.vx * theta, J.vy * theta, J.vz * theta);
+            otx = t.x; oty = t.y; otz = t.z;
+        } else if (J.type == kScrew) {
+            const float th = J.wn * theta;
+            float sh, ch, s, c;
+            sincos_pr(0.5f * th, sh, ch);
+            sincos_pr(th, s, c);
+            float ca, cb;
+            if (fabsf(th) < 1e-3f) {
+                ca = 0.5f * th;
+                cb = (th * th) * 0x1.555556p-3f;
+            } else {
+                ca = (1.0f - c) / th;
+                cb = (th - s) / th;
+            }
+            const float rx = J.vx * theta, ry = J.vy * theta, rz = J.vz * theta;
+            const float cx = __builtin_fmaf(J.ay, rz, -(J.az * ry));
+            const float cy = __builtin_fmaf(J.az, rx, -(J.ax * rz));
+            const float cz = __builtin_fmaf(J.ax, ry, -(J.ay * rx));
+            const float dx = __builtin_fmaf(J.ay, cz, -(J.az * cy));
+            const float dy = __builtin_fmaf(J.az, cx, -(J.ax * cz));
+            const float dz = __builtin_fmaf(J.ax, cy, -(J.ay * cx));
+            const float ex = __builtin_fmaf(cb, dx, __builtin_fmaf(ca, cx, rx));
+            const float ey = __builtin_fmaf(cb, dy, __builtin_fmaf(ca, cy, ry));
+            const float ez = __builtin_fmaf(cb, dz, __builtin_fmaf(ca, cz, rz));
+            const Quat E = {ch, sh * J.ax, sh * J.ay, sh * J.az};
+            oq = qmul(aq, E);
+            const float3 t = mat_apply(quat_mat(aq), atx, aty, atz, ex, ey, ez);
+            otx = t.x; oty = t.y; otz = t.z;
+        }
+        float* To = L.T + (j * 7) * nt;
+        To[0] = oq.w; To[nt] = oq.x; To[2 * nt] = oq.y; To[3 * nt] = oq.z;
+        To[4 * nt] = otx; To[5 * nt] = oty; To[6 * nt] = otz;
+        pq = oq; ptx = otx; pty = oty; ptz = otz;
+    }
+}
+
+// Loads joint js's transform as a rotation matrix + translation.
+__device__ __forceinline__ void load_frame(const LaneScratch& L, int js, Mat3& R, float& tx, float& ty, float& tz) {
+    const int nt = L.nt;
+    const float* T = L.T + (js * 7) * nt;
+    const Quat q = {T[0], T[nt], T[2 * nt], T[3 * nt]};
+    R = quat_mat(q);
+    tx = T[4 * nt]; ty = T[5 * nt]; tz = T[6 * nt];
+}
+
+// ---------------------------------------------------------------------------------- predicates
+// Binary world test of one robot sphere (early exit on the first hit).
+__device__ __forceinline__ bool sphere_hits_world(const Model& M, float sx, float sy, float sz, float sr) {
+    int i = 0;
+    for (; i + 4 <= M.n_ws; i += 4) {
+        bool h = hit_sphere(sx, sy, sz, sr, M.wsph[i]);
+        h |= hit_sphere(sx, sy, sz, sr, M.wsph[i + 1]);
+        h |= hit_sphere(sx, sy, sz, sr, M.wsph[i + 2]);
+        h |= hit_sphere(sx, sy, sz, sr, M.wsph[i + 3]);
+        if (h) return true;
+    }
+    for (; i < M.n_ws; ++i)
+        if (hit_sphere(sx, sy, sz, sr, M.wsph[i])) return true;
+    for (int k = 0; k < M.n_wc; ++k)
+        if (hit_capsule(sx, sy, sz, sr, M.wcap[k])) return true;
+    for (int k = 0; k < M.n_wb; ++k)
+        if (hit_box(sx, sy, sz, sr, M.wbox[k])) return true;
+    for (int k = 0; k < M.n_wh; ++k)
+        if (hit_halfspace(sx, sy, sz, sr, M.whs[k])) return true;
+    return false;
+}
+
+__device__ __forceinline__ float sphere_world_min(const Model& M, float sx, float sy, float sz, float sr, float best) {
+    for (int i = 0; i < M.n_ws; ++i) best = fminf(best, dist_sphere(sx, sy, sz, sr, M.wsph[i]));
+    for (int k = 0; k < M.n_wc; ++k) best = fminf(best, dist_capsule(sx, sy, sz, sr, M.wcap[k]));
+    for (int k = 0; k < M.n_wb; ++k) best = fminf(best, dist_box(sx, sy, sz, sr, M.wbox[k]));
+    for (int k = 0; k < M.n_wh; ++k) best = fminf(best, dist_halfspace(sx, sy, sz, sr, M.whs[k]));
+    return best;
+}
+
+// Work actually done by one predicate call: robot spheres tested against the world and self
+// pairs tested (an early exit counts the sphere it stopped in as fully tested).
+struct Work {
+    unsigned spheres = 0, pairs = 0;
+};
+
+// Binary predicate (prrtc_planner.cu:232-307): any world or self contact.  The config is in
+// L.q; FK goes to L.T.
+__device__ __forceinline__ bool lane_hits(const Model& M, const LaneScratch& L, Work& w) {
+    if (!M.enabled) return false;
+    fk_lane(M, L);
+    int cur = -1;
+    Mat3 R;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    for (int s = 0; s < M.n_spheres; ++s) {
+        const int js = M.sphere_js[s];
+        if (js < 0) continue;
+        if (js != cur) { load_frame(L, js, R, tx, ty, tz); cur = js; }
+        const float4 sp = M.spheres[s];
+        const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+        ++w.spheres;
+        if (sphere_hits_world(M, p.x, p.y, p.z, sp.w)) return true;
+    }
+    int ci = -1, cj = -1;
+    Mat3 Ri, Rj;
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    for (int k = 0; k < M.n_pairs; ++k) {
+        const int2 pr = M.pairs[k];
+        const int ji = M.sphere_js[pr.x], jj = M.sphere_js[pr.y];
+        if (ji != ci) { load_frame(L, ji, Ri, ix, iy, iz); ci = ji; }
+        if (jj != cj) { load_frame(L, jj, Rj, jx, jy, jz); cj = jj; }
+        const float4 a = M.spheres[pr.x];
+        const float4 b = M.spheres[pr.y];
+        const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
+        const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
+        ++w.pairs;
+        if (hit_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w))) return true;
+    }
+    return false;
+}
+
+// Minimum signed distance over every valid pair (FLT_MAX when none), no early exit.
+__device__ __forceinline__ float lane_min_distance(const Model& M, const LaneScratch& L, Work& w) {
+    fk_lane(M, L);
+    float best = FLT_MAX;
+    int cur = -1;
+    Mat3 R;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    for (int s = 0; s < M.n_spheres; ++s) {
+        const int js = M.sphere_js[s];
+        if (js < 0) continue;
+        if (js != cur) { load_frame(L, js, R, tx, ty, tz); cur = js; }
+        const float4 sp = M.spheres[s];
+        const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+        ++w.spheres;
+        best = sphere_world_min(M, p.x, p.y, p.z, sp.w, best);
+    }
+    int ci = -1, cj = -1;
+    Mat3 Ri, Rj;
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    for (int k = 0; k < M.n_pairs; ++k) {
+        const int2 pr = M.pairs[k];
+        const int ji = M.sphere_js[pr.x], jj = M.sphere_js[pr.y];
+        if (ji != ci) { load_frame(L, ji, Ri, ix, iy, iz); ci = ji; }
+        if (jj != cj) { load_frame(L, jj, Rj, jx, jy, jz); cj = jj; }
+        const float4 a = M.spheres[pr.x];
+        const float4 b = M.spheres[pr.y];
+        const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
+        const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
+        ++w.pairs;
+        best = fminf(best, dist_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w)));
+    }
+    return best;
+}
+
+// The planner's collision verdict for the config in L.q (prrtc_planner.cu:501-531).
+__device__ __forceinline__ bool lane_in_collision(const Model& M, const LaneScratch& L, Work& w) {
+    if (!M.enabled) return false;
+    if (M.mode == 1) return lane_min_distance(M, L, w) < M.margin;
+    return lane_hits(M, L, w);
+}
+
+// Algorithmic flops of the work counted in w for one query's model (FMA = 2): FK 128 per
+// revolute joint, 33 per sphere transform, and per sphere-primitive test 10 (sphere), 27
+// (capsule), 36 (box), 9 (halfspace); 10 per self pair.
+__device__ __forceinline__ double work_flops(const Model& M, unsigned long long configs, unsigned long long spheres,
+                                             unsigned long long pairs) {
+    const double world = 10.0 * M.n_ws + 27.0 * M.n_wc + 36.0 * M.n_wb + 9.0 * M.n_wh;
+    const double fk = M.enabled ? 128.0 * M.n_joints : 0.0;
+    return fk * static_cast<double>(configs) + (33.0 + world) * static_cast<double>(spheres) +
+           10.0 * static_cast<double>(pairs);
+}
+
+}  // namespace prrtc

@@ -1,0 +1,270 @@
+// prrtc_kernels.hip -- standalone kernels of the planner path and their C ABI:
+//   prrtc_check_configs       batched FK + collision verdict / min distance (one lane per config)
+//   prrtc_forward_kinematics  batched FK
+//   prrtc_nearest_neighbor    batched nearest node, lowest index on ties (PrrtcNearestNeighborFfi)
+//   prrtc_extend              steer toward samples (PrrtcExtendFfi)
+// All share the planner's device code, so their results are the planner's results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "prrtc_amd.h"
+#include "prrtc_common.h"
+#include "prrtc_host.h"
+
+namespace prrtc {
+
+struct CheckArgs {
+    RobotDev robot;
+    WorldDev world;
+    ModelLayout ml;
+    size_t off_scan;
+    const float* configs;
+    int n, dim, mode;
+    float margin;
+    uint8_t* out_hit;
+    float* out_min;
+    unsigned long long* flops;  // optional: algorithmic flops of the work done
+};
+
+__global__ __launch_bounds__(1024) void check_kernel(CheckArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    int* scan = reinterpret_cast<int*>(lds + a.off_scan);
+    Model M;
+    stage_model(M, lds, a.ml, a.robot, a.world, 0, a.dim, a.mode, a.margin, scan);
+    LaneScratch L;
+    L.q = reinterpret_cast<float*>(lds + a.ml.lane_q) + threadIdx.x;
+    L.T = reinterpret_cast<float*>(lds + a.ml.lane_T) + threadIdx.x;
+    L.nt = blockDim.x;
+    const int stride = gridDim.x * blockDim.x;
+    Work w;
+    unsigned long long configs = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        for (int d = 0; d < a.dim; ++d) L.q[d * L.nt] = a.configs[static_cast<size_t>(i) * a.dim + d];
+        ++configs;
+        if (a.out_min) {
+            const float md = lane_min_distance(M, L, w);
+            a.out_min[i] = md;
+            if (a.out_hit) a.out_hit[i] = (a.mode == 1) ? (M.enabled && md < a.margin) : lane_in_collision(M, L, w);
+        } else {
+            a.out_hit[i] = lane_in_collision(M, L, w) ? 1 : 0;
+        }
+    }
+    if (a.flops && configs) global_add(a.flops, static_cast<unsigned long long>(work_flops(M, configs, w.spheres, w.pairs)));
+}
+
+struct FkArgs {
+    RobotDev robot;
+    WorldDev world;
+    ModelLayout ml;
+    size_t off_scan;
+    const float* configs;
+    int n, dim;
+    float* out;
+};
+
+__global__ __launch_bounds__(256) void fk_kernel(FkArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    int* scan = reinterpret_cast<int*>(lds + a.off_scan);
+    Model M;
+    stage_model(M, lds, a.ml, a.robot, a.world, 0, a.dim, 0, 0.0f, scan);
+    LaneScratch L;
+    L.q = reinterpret_cast<float*>(lds + a.ml.lane_q) + threadIdx.x;
+    L.T = reinterpret_cast<float*>(lds + a.ml.lane_T) + threadIdx.x;
+    L.nt = blockDim.x;
+    const int nj = M.n_joints;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        for (int d = 0; d < a.dim; ++d) L.q[d * L.nt] = a.configs[static_cast<size_t>(i) * a.dim + d];
+        fk_lane(M, L);
+        for (int j = 0; j < nj; ++j) {
+            const int ti = a.robot.topo_inv[j];
+            float* o = a.out + (static_cast<size_t>(i) * nj + j) * 7;
+            if (ti < 0 || ti >= nj) {
+                for (int c = 0; c < 7; ++c) o[c] = __builtin_nanf("");
+                continue;
+            }
+            for (int c = 0; c < 7; ++c) o[c] = L.T[(ti * 7 + c) * L.nt];
+        }
+    }
+}
+
+struct NnArgs {
+    const float* tree;
+    int dim, ncap, size;
+    const float* queries;
+    int batch;
+    int* out_idx;
+    float* out_dist;
+};
+
+template <int MAXD>
+__global__ __launch_bounds__(256) void nn_kernel(NnArgs a) {
+    __shared__ int list[kEC];
+    __shared__ __align__(16) char nnl[sizeof(float) * kEC * kMaxDim + 2 * sizeof(int) * kEC * 16];
+    const int c0 = blockIdx.x * kEC;
+    const int nl = min(kEC, a.batch - c0);
+    if (threadIdx.x < kEC) list[threadIdx.x] = c0 + threadIdx.x;
+    __syncthreads();
+    nn_tiles<MAXD>(a.tree, a.ncap, a.size, a.dim, list, nl, a.queries, a.out_idx, a.out_dist, nnl);
+    if (threadIdx.x < nl) {
+        const int e = c0 + threadIdx.x;
+        a.out_dist[e] = __builtin_sqrtf(a.out_dist[e]);
+    }
+}
+
+__global__ void extend_kernel(const float* tree, int dim, int ncap, const int* nearest, const float* samples, int batch,
+                              float step, float* out, int* parents, int* valid) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    const int n = nearest[b];
+    if (n < 0 || n >= ncap) {
+        for (int d = 0; d < dim; ++d) out[b * dim + d] = samples[b * dim + d];
+        if (parents) parents[b] = -1;
+        if (valid) valid[b] = 0;
+        return;
+    }
+    float dsq = 0.0f;
+    for (int d = 0; d < dim; ++d) {
+        const float df = tree[static_cast<size_t>(d) * ncap + n] - samples[b * dim + d];
+        dsq = __builtin_fmaf(df, df, dsq);
+    }
+    const float nd = __builtin_sqrtf(dsq);
+    const float s = (nd > 1e-8f) ? fminf(1.0f, step / nd) : 1.0f;
+    for (int d = 0; d < dim; ++d) {
+        const float base = tree[static_cast<size_t>(d) * ncap + n];
+        out[b * dim + d] = base + (samples[b * dim + d] - base) * s;
+    }
+    if (parents) parents[b] = n;
+    if (valid) valid[b] = 1;
+}
+
+size_t scan_offset(const ModelLayout& ml) { return ml.end; }
+
+int model_lds(const prrtc_robot& r, const prrtc_world& w, int& nt, int max_lds, ModelLayout& ml, size_t& lds) {
+    for (;;) {
+        ml = model_layout(r.n_joints, r.n_robot_spheres, w.n_self_pairs, w.n_world_spheres, w.n_world_capsules,
+                          w.n_world_boxes, w.n_world_halfspaces, nt);
+        lds = align16(ml.end + sizeof(int) * 20);
+        if (lds <= static_cast<size_t>(max_lds) || nt <= kWave) break;
+        nt -= kWave;
+    }
+    if (lds > static_cast<size_t>(max_lds)) return fail_with(PRRTC_E_LDS, "robot + world exceed the work-group LDS");
+    return PRRTC_OK;
+}
+
+}  // namespace prrtc
+
+using namespace prrtc;
+
+extern "C" {
+
+int prrtc_check_configs(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim, const prrtc_robot* robot,
+                        const prrtc_world* world, int32_t collision_mode, float collision_margin, uint8_t* out_hit,
+                        float* out_min_dist, const prrtc_launch* launch, uint64_t* work_flops) {
+    int rc = check_robot(robot);
+    if (rc) return rc;
+    rc = check_world(world);
+    if (rc) return rc;
+    if (n < 0 || dim < 1 || dim > PRRTC_MAX_DIM) return fail_with(PRRTC_E_INVALID, "bad n or dim");
+    if (!out_hit && !out_min_dist) return fail_with(PRRTC_E_INVALID, "no output requested");
+    if (collision_mode < 0 || collision_mode > 3) return fail_with(PRRTC_E_INVALID, "collision_mode must be 0..3");
+    if (n == 0) return PRRTC_OK;
+    if (!configs) return fail_with(PRRTC_E_INVALID, "configs is null");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail_with(PRRTC_E_HIP, "hipGetDevice failed");
+    const DevInfo& di = dev_info(dev);
+    int nt = (launch && launch->threads > 0) ? launch->threads : 256;
+    if (nt % kWave || nt > 1024) return fail_with(PRRTC_E_INVALID, "threads must be a multiple of 64, <= 1024");
+    CheckArgs a;
+    std::memset(&a, 0, sizeof(a));
+    size_t lds = 0;
+    rc = model_lds(*robot, *world, nt, di.max_lds, a.ml, lds);
+    if (rc) return rc;
+    a.robot = robot_dev(*robot);
+    a.world = world_dev(*world, false);
+    a.off_scan = scan_offset(a.ml);
+    a.configs = configs; a.n = n; a.dim = dim; a.mode = (collision_mode == 1) ? 1 : 0; a.margin = collision_margin;
+    a.out_hit = out_hit; a.out_min = out_min_dist;
+    a.flops = reinterpret_cast<unsigned long long*>(work_flops);
+    int grid = (launch && launch->slots > 0) ? launch->slots : (n + nt - 1) / nt;
+    grid = std::max(1, std::min(grid, 65535));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&check_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    hipLaunchKernelGGL(check_kernel, dim3(grid), dim3(nt), lds, reinterpret_cast<hipStream_t>(stream), a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    return PRRTC_OK;
+}
+
+int prrtc_forward_kinematics(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim,
+                             const prrtc_robot* robot, float* out) {
+    int rc = check_robot(robot);
+    if (rc) return rc;
+    if (n < 0 || dim < 1 || dim > PRRTC_MAX_DIM) return fail_with(PRRTC_E_INVALID, "bad n or dim");
+    if (n == 0 || robot->n_joints == 0) return PRRTC_OK;
+    if (!configs || !out) return fail_with(PRRTC_E_INVALID, "null pointer");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail_with(PRRTC_E_HIP, "hipGetDevice failed");
+    const DevInfo& di = dev_info(dev);
+    prrtc_world empty;
+    std::memset(&empty, 0, sizeof(empty));
+    prrtc_robot rb = *robot;
+    rb.n_robot_spheres = 0;
+    int nt = 256;
+    FkArgs a;
+    std::memset(&a, 0, sizeof(a));
+    size_t lds = 0;
+    rc = model_lds(rb, empty, nt, di.max_lds, a.ml, lds);
+    if (rc) return rc;
+    if (nt != 256) return fail_with(PRRTC_E_LDS, "robot too large for the FK kernel");
+    a.robot = robot_dev(rb);
+    a.world = world_dev(empty, false);
+    a.off_scan = scan_offset(a.ml);
+    a.configs = configs; a.n = n; a.dim = dim; a.out = out;
+    const int grid = std::max(1, std::min((n + nt - 1) / nt, 65535));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fk_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    hipLaunchKernelGGL(fk_kernel, dim3(grid), dim3(nt), lds, reinterpret_cast<hipStream_t>(stream), a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    return PRRTC_OK;
+}
+
+int prrtc_nearest_neighbor(prrtc_stream_t stream, const float* tree_configs, int32_t dim, int32_t max_nodes,
+                           int32_t tree_size, const float* queries, int32_t batch, int32_t* out_idx, float* out_dist) {
+    if (dim < 1 || dim > PRRTC_MAX_DIM || max_nodes < 1 || batch < 0)
+        return fail_with(PRRTC_E_INVALID, "bad dim/max_nodes/batch");
+    if (batch == 0) return PRRTC_OK;
+    if (!tree_configs || !queries || !out_idx || !out_dist) return fail_with(PRRTC_E_INVALID, "null pointer");
+    // The reference clamps the size to [1, max_nodes] (prrtc_nearest_neighbor.cu:113-114).
+    const int size = std::max(1, std::min(tree_size, max_nodes));
+    NnArgs a{tree_configs, dim, max_nodes, size, queries, batch, out_idx, out_dist};
+    const int grid = (batch + kEC - 1) / kEC;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dim <= 8) hipLaunchKernelGGL(nn_kernel<8>, dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(nn_kernel<16>, dim3(grid), dim3(256), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    return PRRTC_OK;
+}
+
+int prrtc_extend(prrtc_stream_t stream, const float* tree_configs, int32_t dim, int32_t max_nodes,
+                 const int32_t* nearest, const float* samples, int32_t batch, float step_size, float* out_configs,
+                 int32_t* out_parents, int32_t* out_valid) {
+    if (dim < 1 || dim > PRRTC_MAX_DIM || max_nodes < 1 || batch < 0)
+        return fail_with(PRRTC_E_INVALID, "bad dim/max_nodes/batch");
+    if (batch == 0) return PRRTC_OK;
+    if (!tree_configs || !nearest || !samples || !out_configs) return fail_with(PRRTC_E_INVALID, "null pointer");
+    hipLaunchKernelGGL(extend_kernel, dim3((batch + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       tree_configs, dim, max_nodes, nearest, samples, batch, step_size, out_configs, out_parents,
+                       out_valid);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    return PRRTC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,590 @@
+"""pRRTC planner API for MI355X -- the drop-in for ``cuda-rrtc/jax/prrtc.py``.
+
+``prrtc_plan``, ``prrtc_plan_batch``, ``prrtc_nearest_neighbor``, ``prrtc_extend`` and
+``PRRTCResult`` keep the reference's names, keyword arguments, defaults, return types and
+``ValueError`` / ``RuntimeError`` conditions (prrtc.py:31-45, 54-73, 255-585, 882-1508).  Host
+arrays may be numpy, lists or torch tensors; device buffers are torch tensors on the current
+HIP device, and every planner call is one enqueue of the C ABI in ``include/prrtc_amd.h`` on
+torch's current stream.
+
+Differences from the reference (each a documented extension or fix):
+
+* ``PRRTCResult.path`` is a host ``numpy`` array; tree fields are device tensors.
+* Plans are deterministic: the explorers run the reference kernel's algorithm in lockstep
+  rounds, so the same inputs (and ``seed``) always give the same trees.
+* ``prrtc_plan_batch`` accepts uneven goal sets (a list of ``[G_i, dim]`` arrays),
+  per-query ``max_nodes`` and per-query ``seeds``; all queries run in one persistent launch.
+* ``status`` (solved / iteration cap / tree full) is reported per query.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import NamedTuple, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .context import (DeviceRobot, DeviceWorld, as_numpy, check_context, pack_worlds, upload_robot,
+                      upload_world, REQUIRED_KEYS)
+
+
+class PRRTCResult(NamedTuple):
+    """Result of pRRTC planning (prrtc.py:31-45)."""
+
+    solved: bool
+    path: Optional[np.ndarray]
+    tree_a_size: int
+    tree_b_size: int
+    iterations: int
+    cost: float
+    kernel_time_ms: Optional[float] = None
+    tree_a_configs: Optional[torch.Tensor] = None
+    tree_b_configs: Optional[torch.Tensor] = None
+    tree_a_parents: Optional[torch.Tensor] = None
+    tree_b_parents: Optional[torch.Tensor] = None
+    status: int = 0
+
+
+_COLLISION_MODES = {"binary": 0, "differentiable": 1, "binary_coarse": 2, "binary_coarse_coop": 3}
+DEFAULT_PATH_CAPACITY = 4096
+
+
+def _resolve_collision_mode(collision_checker: str) -> int:
+    try:
+        return _COLLISION_MODES[collision_checker]
+    except KeyError:
+        if collision_checker == "robogpu":
+            raise ValueError(
+                "collision_checker='robogpu' (OptiX point-cloud checking) is not available in the "
+                "in-kernel planner; use one of " + str(sorted(_COLLISION_MODES))
+            ) from None
+        raise ValueError(
+            f"collision_checker must be one of {sorted(_COLLISION_MODES)}, got {collision_checker!r}"
+        ) from None
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("prrax_amd needs a HIP device (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _check_granularity(granularity: int) -> int:
+    granularity = int(granularity)
+    if granularity < 1 or (granularity & (granularity - 1)) != 0:
+        raise ValueError(f"granularity must be a power of 2, got {granularity}")
+    if granularity > 64:
+        raise ValueError(f"granularity must be <= 64 (PRRTC_BLOCK_THREADS_MAX), got {granularity}")
+    return granularity
+
+
+def _attrs(dim, max_iterations, step_size, num_new_samples, granularity, max_nodes, balance_mode, tree_ratio,
+           dynamic_domain, dd_alpha, dd_radius, dd_min_radius, collision_mode, collision_margin) -> _lib.Attrs:
+    return _lib.Attrs(int(max_iterations), float(step_size), int(num_new_samples), int(balance_mode),
+                      float(tree_ratio), 1 if dynamic_domain else 0, float(dd_alpha), float(dd_radius),
+                      float(dd_min_radius), int(dim), int(max_nodes), int(granularity), int(collision_mode),
+                      float(collision_margin))
+
+
+def _empty_context(dim: int) -> dict:
+    z = np.zeros
+    return {
+        "fk_twists": z((0, 6), np.float32), "fk_parent_tf": z((0, 7), np.float32),
+        "fk_parent_idx": z(0, np.int32), "fk_act_idx": z(0, np.int32), "fk_mimic_mul": z(0, np.float32),
+        "fk_mimic_off": z(0, np.float32), "fk_mimic_act_idx": z(0, np.int32), "fk_topo_inv": z(0, np.int32),
+        "sphere_link_idx": z(0, np.int32), "sphere_local": z((0, 3), np.float32), "sphere_radius": z(0, np.float32),
+    }
+
+
+class PlanBuffers:
+    """Device outputs + workspace of one planner launch over ``batch`` queries."""
+
+    N_INT = 11  # sizes 2 | completed 2 | iter 1 | conn 3 | solved 1 | status 1 | path_len 1
+
+    def __init__(self, batch: int, dim: int, max_nodes: int, num_new_samples: int, path_capacity: int, device):
+        self.batch, self.dim, self.max_nodes, self.path_capacity = batch, dim, max_nodes, path_capacity
+        f32, i32 = torch.float32, torch.int32
+        self.tree_a = torch.empty((batch, dim, max_nodes), dtype=f32, device=device)
+        self.tree_b = torch.empty((batch, dim, max_nodes), dtype=f32, device=device)
+        self.par_a = torch.empty((batch, max_nodes), dtype=i32, device=device)
+        self.par_b = torch.empty((batch, max_nodes), dtype=i32, device=device)
+        self.ints = torch.zeros(self.N_INT * batch, dtype=i32, device=device)
+        self.times = torch.zeros(batch, dtype=f32, device=device)
+        self.path = torch.empty((batch, path_capacity, dim), dtype=f32, device=device)
+        nbytes = _lib.lib().prrtc_planner_workspace_bytes(batch, dim, max_nodes, num_new_samples)
+        if nbytes == 0:
+            raise ValueError("invalid planner sizes")
+        self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+        self.stats_offset = int(_lib.lib().prrtc_planner_stats_offset(batch, dim, max_nodes, num_new_samples))
+
+    def stats(self) -> dict:
+        """Counters of the last launch: configs checked, rounds, algorithmic flops."""
+        raw = self.workspace[self.stats_offset:self.stats_offset + 32].cpu().numpy().view(np.uint64)
+        return {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2])}
+
+    def outputs(self) -> _lib.Outputs:
+        B = self.batch
+        base = self.ints.data_ptr()
+        off = lambda k: base + 4 * k * B  # noqa: E731
+        return _lib.Outputs(
+            self.tree_a.data_ptr(), self.tree_b.data_ptr(), self.par_a.data_ptr(), self.par_b.data_ptr(),
+            off(0), off(2), off(4), off(5), off(8), self.times.data_ptr(), off(9), self.path.data_ptr(), off(10),
+            self.path_capacity,
+        )
+
+    def host_summary(self) -> dict:
+        B = self.batch
+        ints = self.ints.cpu().numpy()
+        return {
+            "sizes": ints[0:2 * B].reshape(B, 2),
+            "completed": ints[2 * B:4 * B].reshape(B, 2),
+            "iterations": ints[4 * B:5 * B],
+            "conn": ints[5 * B:8 * B].reshape(B, 3),
+            "solved": ints[8 * B:9 * B],
+            "status": ints[9 * B:10 * B],
+            "path_len": ints[10 * B:11 * B],
+            "kernel_time_ms": self.times.cpu().numpy(),
+        }
+
+    def results(self) -> list[PRRTCResult]:
+        h = self.host_summary()
+        longest = int(max(h["path_len"].max(initial=0), 0))
+        paths = self.path[:, :longest].cpu().numpy() if longest else None
+        out = []
+        for i in range(self.batch):
+            solved = bool(h["solved"][i] == 1)
+            sa, sb = int(h["sizes"][i, 0]), int(h["sizes"][i, 1])
+            path, cost = None, float("inf")
+            if solved:
+                n = int(h["path_len"][i])
+                if n > 0:
+                    path = paths[i, :n].copy()
+                else:
+                    path = trace_path_host(self, i, h["conn"][i])
+                cost = float(np.sum(np.linalg.norm(np.diff(path, axis=0), axis=1)))
+            out.append(PRRTCResult(
+                solved=solved, path=path, tree_a_size=sa, tree_b_size=sb, iterations=int(h["iterations"][i]),
+                cost=cost, kernel_time_ms=float(h["kernel_time_ms"][i]),
+                tree_a_configs=self.tree_a[i, :, :sa], tree_b_configs=self.tree_b[i, :, :sb],
+                tree_a_parents=self.par_a[i, :sa], tree_b_parents=self.par_b[i, :sb], status=int(h["status"][i]),
+            ))
+        return out
+
+
+def trace_path_host(bufs: PlanBuffers, i: int, conn) -> np.ndarray:
+    """Host walk used only when a path exceeds the device path buffer (prrtc.py:588-640)."""
+    pa = bufs.par_a[i].cpu().numpy()
+    pb = bufs.par_b[i].cpu().numpy()
+    ta = bufs.tree_a[i].cpu().numpy()
+    tb = bufs.tree_b[i].cpu().numpy()
+    a_nodes, cur = [], int(conn[0])
+    for _ in range(bufs.max_nodes):
+        a_nodes.append(ta[:, cur])
+        if pa[cur] == cur:
+            break
+        cur = int(pa[cur])
+    a_nodes.reverse()
+    b_nodes, cur = [], int(conn[1])
+    for _ in range(bufs.max_nodes):
+        b_nodes.append(tb[:, cur])
+        if pb[cur] == cur:
+            break
+        cur = int(pb[cur])
+    return np.stack(a_nodes + b_nodes, axis=0).astype(np.float32)
+
+
+@dataclass
+class PreparedBatch:
+    """Everything one planner launch needs, already on the device."""
+
+    robot: DeviceRobot
+    world: DeviceWorld
+    starts: torch.Tensor
+    goals: torch.Tensor
+    goal_counts: Optional[torch.Tensor]
+    min_vals: torch.Tensor
+    max_vals: torch.Tensor
+    max_nodes_q: Optional[torch.Tensor]
+    seeds: Optional[torch.Tensor]
+    order: Optional[torch.Tensor]
+    attrs: _lib.Attrs
+    bufs: PlanBuffers
+    launch: Optional[_lib.Launch] = None
+
+    @property
+    def batch(self) -> int:
+        return int(self.starts.shape[0])
+
+    def enqueue(self) -> None:
+        L = _lib.lib()
+        ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        out = self.bufs.outputs()
+        launch = self.launch if self.launch is not None else _lib.Launch(0, 0)
+        rc = L.prrtc_planner_batch(
+            _stream_ptr(), self.batch, self.starts.data_ptr(), self.goals.data_ptr(), int(self.goals.shape[1]),
+            ptr(self.goal_counts), self.min_vals.data_ptr(), self.max_vals.data_ptr(), self.robot.struct,
+            self.world.struct, 1 if self.world.per_query else 0, ptr(self.max_nodes_q), ptr(self.seeds),
+            ptr(self.order), self.attrs, out, launch, self.bufs.workspace.data_ptr(), self.bufs.workspace.numel())
+        _lib.check("prrtc_planner_batch", rc)
+
+
+def _limits(min_vals, max_vals, dim, device):
+    lo = np.full(dim, -np.pi, np.float32) if min_vals is None else as_numpy(min_vals, np.float32).reshape(dim)
+    hi = np.full(dim, np.pi, np.float32) if max_vals is None else as_numpy(max_vals, np.float32).reshape(dim)
+    return torch.from_numpy(lo).to(device), torch.from_numpy(hi).to(device)
+
+
+def prrtc_plan(
+    start_config,
+    goal_configs,
+    max_iterations: int = 1_000_000,
+    step_size: float = 0.5,
+    num_new_samples: int = 128,
+    granularity: int = 16,
+    max_nodes: int = 1_000_000,
+    balance_mode: int = 1,
+    tree_ratio: float = 0.5,
+    dynamic_domain: bool = True,
+    dd_alpha: float = 1e-4,
+    dd_radius: float = 4.0,
+    dd_min_radius: float = 1.0,
+    min_vals=None,
+    max_vals=None,
+    collision_context: Optional[dict] = None,
+    allow_unsafe_no_collision: bool = False,
+    jit_trace: bool = False,
+    collision_checker: str = "binary",
+    collision_margin: float = 0.02,
+    *,
+    seed: int = 0,
+    path_capacity: int = DEFAULT_PATH_CAPACITY,
+    threads: int = 0,
+) -> PRRTCResult:
+    """Plan one query (prrtc.py:255-585).  ``jit_trace`` is accepted for compatibility: the
+    dispatch is already a single C call.  ``seed`` (extension) selects the sampler sequence;
+    ``0`` reproduces the reference's per-block Halton sequences."""
+    collision_mode = _resolve_collision_mode(collision_checker)
+    _lib.lib()
+    start = as_numpy(start_config, np.float32)
+    start = np.atleast_1d(start)
+    if start.ndim == 1:
+        start = start.reshape(1, -1)
+    if start.shape[0] != 1:
+        raise ValueError(
+            "prrtc_plan currently accepts a single start configuration per call. "
+            "For batches, use prrtc_plan_batch."
+        )
+    dim = int(start.shape[-1])
+    goals = as_numpy(goal_configs, np.float32).reshape(-1, dim)
+    step_size = float(step_size)
+    if step_size <= 0.0:
+        raise ValueError("step_size must be > 0")
+    granularity = _check_granularity(granularity)
+    if collision_context is None and not allow_unsafe_no_collision:
+        raise ValueError(
+            "Collision-aware planning is the default. Provide collision_context "
+            "with required keys or set allow_unsafe_no_collision=True to opt out. "
+            f"Required keys: {REQUIRED_KEYS}"
+        )
+    arrays = check_context(collision_context if collision_context is not None else _empty_context(dim), dim)
+    dev = _device()
+    robot = upload_robot(arrays, dev)
+    world = upload_world(arrays, dev)
+    lo, hi = _limits(min_vals, max_vals, dim, dev)
+    attrs = _attrs(dim, max_iterations, step_size, num_new_samples, granularity, max_nodes, balance_mode, tree_ratio,
+                   dynamic_domain, dd_alpha, dd_radius, dd_min_radius, collision_mode, collision_margin)
+    bufs = PlanBuffers(1, dim, int(max_nodes), int(num_new_samples), int(path_capacity), dev)
+    starts_t = torch.from_numpy(start.reshape(1, dim)).to(dev)
+    goals_t = torch.from_numpy(np.ascontiguousarray(goals.reshape(1, -1, dim))).to(dev)
+    seeds_t = None
+    if seed:
+        seeds_t = torch.from_numpy(np.array([int(seed) & 0xFFFFFFFF], np.uint32).view(np.int32)).to(dev)
+    if seed == 0:
+        L = _lib.lib()
+        out = bufs.outputs()
+        rc = L.prrtc_planner(_stream_ptr(), starts_t.data_ptr(), goals_t.data_ptr(), int(goals.shape[0]),
+                             lo.data_ptr(), hi.data_ptr(), robot.struct, world.struct, attrs, out,
+                             _lib.Launch(int(threads), 0), bufs.workspace.data_ptr(), bufs.workspace.numel())
+        _lib.check("prrtc_planner", rc)
+    else:
+        PreparedBatch(robot, world, starts_t, goals_t, None, lo, hi, None, seeds_t, None, attrs, bufs,
+                      _lib.Launch(int(threads), 0)).enqueue()
+    return bufs.results()[0]
+
+
+def prepare_batch(
+    start_configs,
+    goal_configs,
+    *,
+    collision_context=None,
+    allow_unsafe_no_collision: bool = False,
+    max_iterations: int = 1_000_000,
+    step_size: float = 0.5,
+    num_new_samples: int = 128,
+    granularity: int = 16,
+    max_nodes: int = 1_000_000,
+    balance_mode: int = 1,
+    tree_ratio: float = 0.5,
+    dynamic_domain: bool = True,
+    dd_alpha: float = 1e-4,
+    dd_radius: float = 4.0,
+    dd_min_radius: float = 1.0,
+    min_vals=None,
+    max_vals=None,
+    collision_checker: str = "binary",
+    collision_margin: float = 0.02,
+    max_nodes_per_query=None,
+    seeds=None,
+    order=None,
+    path_capacity: int = DEFAULT_PATH_CAPACITY,
+    threads: int = 0,
+    slots: int = 0,
+) -> PreparedBatch:
+    """Validates and uploads a batch; ``PreparedBatch.enqueue()`` then runs it (repeatably)."""
+    collision_mode = _resolve_collision_mode(collision_checker)
+    _lib.lib()
+    starts = np.atleast_2d(as_numpy(start_configs, np.float32))
+    B, dim = int(starts.shape[0]), int(starts.shape[1])
+    if isinstance(goal_configs, (list, tuple)):
+        glist = [as_numpy(g, np.float32).reshape(-1, dim) for g in goal_configs]
+        if len(glist) != B:
+            raise ValueError(f"goal_configs list must have length batch_size ({B}), got {len(glist)}")
+        gmax = max(g.shape[0] for g in glist)
+        if min(g.shape[0] for g in glist) < 1:
+            raise ValueError("every query needs at least one goal")
+        goals = np.zeros((B, gmax, dim), np.float32)
+        for i, g in enumerate(glist):
+            goals[i, : g.shape[0]] = g
+        gcounts = np.array([g.shape[0] for g in glist], np.int32)
+    else:
+        goals = as_numpy(goal_configs, np.float32)
+        if goals.ndim == 2:
+            goals = goals[:, None, :]
+        if goals.ndim != 3 or goals.shape[0] != B:
+            raise ValueError(
+                f"goal_configs must have shape (batch, num_goals, dim), got {goals.shape} with batch_size={B}")
+        gcounts = None
+    step_size = float(step_size)
+    if step_size <= 0.0:
+        raise ValueError("step_size must be > 0")
+    granularity = _check_granularity(granularity)
+    dev = _device()
+    if isinstance(collision_context, (list, tuple)):
+        if len(collision_context) != B:
+            raise ValueError(
+                "When collision_context is a list/tuple, it must have length "
+                f"equal to batch_size ({B}), got {len(collision_context)}")
+        first = next((c for c in collision_context if c is not None), None)
+        if first is None and not allow_unsafe_no_collision:
+            raise ValueError(
+                "Collision-aware planning is the default. Provide collision_context "
+                "with required keys or set allow_unsafe_no_collision=True to opt out.")
+        base = check_context(first if first is not None else _empty_context(dim), dim)
+        worlds = []
+        for i, c in enumerate(collision_context):
+            if c is None:
+                worlds.append(check_context({**_empty_context(dim), **{k: base[k] for k in REQUIRED_KEYS}}, dim))
+            else:
+                try:
+                    worlds.append(check_context(c, dim))
+                except ValueError as e:
+                    raise ValueError(f"collision_context[{i}]: {e}") from None
+        robot = upload_robot(base, dev)
+        world = pack_worlds(worlds, dev)
+    else:
+        if collision_context is None and not allow_unsafe_no_collision:
+            raise ValueError(
+                "Collision-aware planning is the default. Provide collision_context "
+                "with required keys or set allow_unsafe_no_collision=True to opt out. "
+                f"Required keys: {REQUIRED_KEYS}")
+        arrays = check_context(collision_context if collision_context is not None else _empty_context(dim), dim)
+        robot = upload_robot(arrays, dev)
+        world = upload_world(arrays, dev)
+    lo, hi = _limits(min_vals, max_vals, dim, dev)
+    ncap = int(max_nodes)
+    mnq = None
+    if max_nodes_per_query is not None:
+        m = as_numpy(max_nodes_per_query, np.int32).reshape(B)
+        if np.any(m < 1):
+            raise ValueError("max_nodes_per_query entries must be >= 1")
+        ncap = int(m.max())
+        mnq = torch.from_numpy(m).to(dev)
+    sd = None
+    if seeds is not None:
+        s = np.asarray(seeds, dtype=np.int64).reshape(B) & 0xFFFFFFFF
+        sd = torch.from_numpy(s.astype(np.uint32).view(np.int32)).to(dev)
+    od = None
+    if order is not None:
+        o = as_numpy(order, np.int32).reshape(B)
+        if not np.array_equal(np.sort(o), np.arange(B)):
+            raise ValueError("order must be a permutation of range(batch)")
+        od = torch.from_numpy(o).to(dev)
+    attrs = _attrs(dim, max_iterations, step_size, num_new_samples, granularity, ncap, balance_mode, tree_ratio,
+                   dynamic_domain, dd_alpha, dd_radius, dd_min_radius, collision_mode, collision_margin)
+    bufs = PlanBuffers(B, dim, ncap, int(num_new_samples), int(path_capacity), dev)
+    return PreparedBatch(
+        robot, world, torch.from_numpy(np.ascontiguousarray(starts)).to(dev),
+        torch.from_numpy(np.ascontiguousarray(goals)).to(dev),
+        None if gcounts is None else torch.from_numpy(gcounts).to(dev), lo, hi, mnq, sd, od, attrs, bufs,
+        _lib.Launch(int(threads), int(slots)))
+
+
+def prrtc_plan_batch(
+    start_configs,
+    goal_configs,
+    max_iterations: int = 1_000_000,
+    step_size: float = 0.5,
+    num_new_samples: int = 128,
+    granularity: int = 16,
+    max_nodes: int = 1_000_000,
+    balance_mode: int = 1,
+    tree_ratio: float = 0.5,
+    dynamic_domain: bool = True,
+    dd_alpha: float = 1e-4,
+    dd_radius: float = 4.0,
+    dd_min_radius: float = 1.0,
+    min_vals=None,
+    max_vals=None,
+    collision_context=None,
+    allow_unsafe_no_collision: bool = False,
+    jit_trace: bool = False,
+    collision_checker: str = "binary",
+    collision_margin: float = 0.02,
+    *,
+    max_nodes_per_query=None,
+    seeds=None,
+    order=None,
+    path_capacity: int = DEFAULT_PATH_CAPACITY,
+    threads: int = 0,
+    slots: int = 0,
+) -> list[PRRTCResult]:
+    """Plan a batch of independent queries in one persistent launch (prrtc.py:882-1434).
+
+    ``collision_context`` is one shared dict or a list of ``batch`` dicts (per-query worlds and
+    self pairs; the robot comes from the first non-None entry, as in the reference).
+    ``goal_configs`` is ``[batch, num_goals, dim]`` or ``[batch, dim]``, or (extension) a list of
+    ``[G_i, dim]`` arrays for uneven goal sets."""
+    pb = prepare_batch(
+        start_configs, goal_configs, collision_context=collision_context,
+        allow_unsafe_no_collision=allow_unsafe_no_collision, max_iterations=max_iterations, step_size=step_size,
+        num_new_samples=num_new_samples, granularity=granularity, max_nodes=max_nodes, balance_mode=balance_mode,
+        tree_ratio=tree_ratio, dynamic_domain=dynamic_domain, dd_alpha=dd_alpha, dd_radius=dd_radius,
+        dd_min_radius=dd_min_radius, min_vals=min_vals, max_vals=max_vals, collision_checker=collision_checker,
+        collision_margin=collision_margin, max_nodes_per_query=max_nodes_per_query, seeds=seeds, order=order,
+        path_capacity=path_capacity, threads=threads, slots=slots)
+    pb.enqueue()
+    return pb.bufs.results()
+
+
+# ------------------------------------------------------------------------------ config checks
+class CollisionChecker:
+    """Batched FK + collision verdicts on the device for one robot + world."""
+
+    def __init__(self, collision_context: dict, dim: int, collision_checker: str = "binary",
+                 collision_margin: float = 0.02, threads: int = 0):
+        self.mode = _resolve_collision_mode(collision_checker)
+        self.margin = float(collision_margin)
+        self.dim = int(dim)
+        self.arrays = check_context(collision_context, self.dim)
+        dev = _device()
+        self.device = dev
+        self.robot = upload_robot(self.arrays, dev)
+        self.world = upload_world(self.arrays, dev)
+        self.launch = _lib.Launch(int(threads), 0)
+
+    def _configs(self, configs) -> torch.Tensor:
+        if isinstance(configs, torch.Tensor) and configs.is_cuda:
+            return configs.to(torch.float32).contiguous().reshape(-1, self.dim)
+        return torch.from_numpy(as_numpy(configs, np.float32).reshape(-1, self.dim)).to(self.device)
+
+    def enqueue(self, q: torch.Tensor, hit: Optional[torch.Tensor], mind: Optional[torch.Tensor],
+                flops: Optional[torch.Tensor] = None) -> None:
+        """Enqueues one check of q; ``flops`` (int64 device scalar) accumulates the algorithmic
+        flops of the work done."""
+        rc = _lib.lib().prrtc_check_configs(
+            _stream_ptr(), q.data_ptr(), int(q.shape[0]), self.dim, self.robot.struct, self.world.struct, self.mode,
+            self.margin, None if hit is None else hit.data_ptr(), None if mind is None else mind.data_ptr(),
+            self.launch, None if flops is None else flops.data_ptr())
+        _lib.check("prrtc_check_configs", rc)
+
+    def in_collision(self, configs) -> torch.Tensor:
+        q = self._configs(configs)
+        hit = torch.empty(q.shape[0], dtype=torch.uint8, device=self.device)
+        self.enqueue(q, hit, None)
+        return hit.bool()
+
+    def min_distance(self, configs) -> torch.Tensor:
+        q = self._configs(configs)
+        mind = torch.empty(q.shape[0], dtype=torch.float32, device=self.device)
+        self.enqueue(q, None, mind)
+        return mind
+
+    def is_free(self, configs) -> np.ndarray:
+        return (~self.in_collision(configs)).cpu().numpy()
+
+
+def forward_kinematics(collision_context: dict, configs, dim: Optional[int] = None) -> torch.Tensor:
+    """World pose ``[n, n_joints, 7]`` (w, x, y, z, tx, ty, tz) of every joint frame."""
+    q = as_numpy(configs, np.float32)
+    dim = int(dim or q.shape[-1])
+    q = q.reshape(-1, dim)
+    arrays = check_context(collision_context, dim)
+    dev = _device()
+    robot = upload_robot(arrays, dev)
+    qt = torch.from_numpy(q).to(dev)
+    out = torch.empty((q.shape[0], robot.n_joints, 7), dtype=torch.float32, device=dev)
+    rc = _lib.lib().prrtc_forward_kinematics(_stream_ptr(), qt.data_ptr(), int(q.shape[0]), dim, robot.struct,
+                                             out.data_ptr())
+    _lib.check("prrtc_forward_kinematics", rc)
+    return out
+
+
+# ------------------------------------------------------------------------------ NN / extend
+def _dev_tensor(x, dtype) -> torch.Tensor:
+    dev = _device()
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=dtype).contiguous()
+    return torch.from_numpy(as_numpy(x, np.float32 if dtype == torch.float32 else np.int32)).to(dev)
+
+
+def prrtc_nearest_neighbor(tree_configs, query_configs, tree_size: int):
+    """Nearest tree node per query (prrtc.py:1437-1469): returns ``(dists, indices)``;
+    ties go to the lowest index."""
+    tree = _dev_tensor(tree_configs, torch.float32)
+    qs = _dev_tensor(query_configs, torch.float32)
+    dim, max_nodes = int(tree.shape[0]), int(tree.shape[1])
+    qs = qs.reshape(-1, dim)
+    idx = torch.empty(qs.shape[0], dtype=torch.int32, device=tree.device)
+    dist = torch.empty(qs.shape[0], dtype=torch.float32, device=tree.device)
+    rc = _lib.lib().prrtc_nearest_neighbor(_stream_ptr(), tree.data_ptr(), dim, max_nodes, int(tree_size),
+                                           qs.data_ptr(), int(qs.shape[0]), idx.data_ptr(), dist.data_ptr())
+    _lib.check("prrtc_nearest_neighbor", rc)
+    return dist, idx
+
+
+def prrtc_extend(tree_configs, nearest_indices, samples, step_size: float):
+    """Steer from tree nodes toward samples (prrtc.py:1472-1508): returns
+    ``(new_configs, parent_indices, valid_flags)``."""
+    tree = _dev_tensor(tree_configs, torch.float32)
+    nn = _dev_tensor(nearest_indices, torch.int32).reshape(-1)
+    dim, max_nodes = int(tree.shape[0]), int(tree.shape[1])
+    smp = _dev_tensor(samples, torch.float32).reshape(-1, dim)
+    B = int(smp.shape[0])
+    out = torch.empty((B, dim), dtype=torch.float32, device=tree.device)
+    par = torch.empty(B, dtype=torch.int32, device=tree.device)
+    val = torch.empty(B, dtype=torch.int32, device=tree.device)
+    rc = _lib.lib().prrtc_extend(_stream_ptr(), tree.data_ptr(), dim, max_nodes, nn.data_ptr(), smp.data_ptr(), B,
+                                 float(step_size), out.data_ptr(), par.data_ptr(), val.data_ptr())
+    _lib.check("prrtc_extend", rc)
+    return out, par, val
+
+
+def path_cost(path: np.ndarray) -> float:
+    return float(np.sum(np.linalg.norm(np.diff(path, axis=0), axis=1))) if path is not None else math.inf

@@ -1,0 +1,307 @@
+"""Seeded synthetic robots, worlds and queries for the planner workloads.
+
+Every array follows the reference ``collision_context`` wire format
+(README.md:77-104, cuda-rrtc/jax/utils.py:119-252): FK arrays in joint (topological) order,
+fine spheres sorted by FK frame, world primitives in the reference row layouts
+(spheres ``[x,y,z,r]``, capsules ``[p1,p2,r]``, boxes ``[c,a1,a2,a3,hl]``, halfspaces
+``[n,p]``).  All randomness is numpy PCG64 (``np.random.default_rng``) with fixed seeds, so a
+workload is fully named by its builder and seed.
+
+Workloads (sizes as quoted for the benchmark):
+
+* ``c1`` -- 7-DOF serial arm, revolute axes z,y,z,y,z,y,z, link offsets U(0.10, 0.35) m along
+  local z, limits +-2.8 rad, 48 spheres (7 per link, 6 on the last), 60 self pairs; world of
+  40 spheres (r U(0.05, 0.15)) in a 1.2x1.2x1.0 m box in front of the base, plus a floor.
+* ``c2`` -- same arm with 64 spheres / 200 self pairs; 200 spheres (r U(0.02, 0.12)), 16 oriented
+  boxes, 8 capsules, floor; 4 goals.
+* ``c3`` -- per query: 10-300 spheres, 0-24 boxes, floor, 1-8 goals, max_nodes 8192-65536 in
+  proportion to obstacle count.
+* ``c4`` -- 14-DOF dual arm (two ``c2`` arms 0.6 m apart) with a 2-finger gripper per arm whose
+  finger joints mimic the wrist joint (mul 1, off 0): 18 joints, 128 spheres, ~2000 self pairs;
+  world of 150 spheres, 12 boxes, floor.
+* ``c5`` -- the ``c1`` arm against 100 world spheres (path refinement workload).
+
+Start and goal configurations are rejection-sampled: ``sample_free`` takes a checker callback
+``is_free(configs[n, dim]) -> bool[n]`` so that the same sampling runs against any checker.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+AXES_7 = ("z", "y", "z", "y", "z", "y", "z")
+_AXIS = {"x": (1.0, 0.0, 0.0), "y": (0.0, 1.0, 0.0), "z": (0.0, 0.0, 1.0)}
+LIMIT = 2.8
+# Obstacle region in front of the base (x forward).
+REGION_LO = np.array([0.3, -0.6, 0.0])
+REGION_HI = np.array([1.5, 0.6, 1.0])
+
+
+@dataclass
+class Robot:
+    """A robot in the reference FK/sphere wire format plus its joint limits."""
+
+    ctx: dict
+    lower: np.ndarray
+    upper: np.ndarray
+
+    @property
+    def dim(self) -> int:
+        return int(self.lower.shape[0])
+
+
+@dataclass
+class Scene:
+    robot: Robot
+    world: dict
+    extra: dict = field(default_factory=dict)
+
+    def context(self) -> dict:
+        """Full ``collision_context`` (robot keys + world keys)."""
+        out = dict(self.robot.ctx)
+        out.update(self.world)
+        return out
+
+
+def _arm_chain(rng, n_spheres, axes=AXES_7):
+    n = len(axes)
+    lengths = rng.uniform(0.10, 0.35, size=n + 1)  # offsets of joints 0..n-1, then the tool
+    per = [n_spheres // n] * n
+    for k in range(n_spheres - sum(per)):
+        per[k] += 1
+    return lengths, per
+
+
+def _arm_spheres(rng, per, lengths, r_lo, r_hi, link_base=0):
+    links, local, radius = [], [], []
+    for j, cnt in enumerate(per):
+        zs = rng.uniform(0.0, lengths[j + 1], size=cnt)
+        rs = rng.uniform(r_lo, r_hi, size=cnt)
+        for z, r in zip(zs, rs):
+            links.append(link_base + j)
+            local.append((0.0, 0.0, z))
+            radius.append(r)
+    return links, local, radius
+
+
+def _pick_pairs(rng, cand, n_pairs):
+    cand = np.asarray(cand, dtype=np.int32).reshape(-1, 2)
+    n_pairs = min(n_pairs, cand.shape[0])
+    sel = np.sort(rng.choice(cand.shape[0], size=n_pairs, replace=False))
+    return cand[sel]
+
+
+def make_arm(seed: int = 0, n_spheres: int = 48, n_self_pairs: int = 60) -> Robot:
+    """7-DOF serial arm (axes z,y,z,y,z,y,z), base at the origin."""
+    rng = np.random.default_rng(seed)
+    lengths, per = _arm_chain(rng, n_spheres)
+    n = len(AXES_7)
+    twists = np.zeros((n, 6), np.float32)
+    parent_tf = np.zeros((n, 7), np.float32)
+    for j, a in enumerate(AXES_7):
+        twists[j, 3:] = _AXIS[a]
+        parent_tf[j] = (1.0, 0.0, 0.0, 0.0, 0.0, 0.0, lengths[j])
+    links, local, radius = _arm_spheres(rng, per, lengths, 0.04, 0.08)
+    links = np.asarray(links, np.int32)
+    cand = [(i, k) for i in range(len(links)) for k in range(i + 1, len(links))
+            if abs(int(links[i]) - int(links[k])) >= 2]
+    pairs = _pick_pairs(rng, cand, n_self_pairs)
+    ctx = {
+        "fk_twists": twists,
+        "fk_parent_tf": parent_tf,
+        "fk_parent_idx": np.arange(-1, n - 1, dtype=np.int32),
+        "fk_act_idx": np.arange(n, dtype=np.int32),
+        "fk_mimic_mul": np.ones(n, np.float32),
+        "fk_mimic_off": np.zeros(n, np.float32),
+        "fk_mimic_act_idx": np.full(n, -1, np.int32),
+        "fk_topo_inv": np.arange(n, dtype=np.int32),
+        "sphere_link_idx": links,
+        "sphere_local": np.asarray(local, np.float32),
+        "sphere_radius": np.asarray(radius, np.float32),
+        "self_pairs": pairs.astype(np.int32),
+    }
+    return Robot(ctx, np.full(n, -LIMIT, np.float32), np.full(n, LIMIT, np.float32))
+
+
+def make_dual_arm(seed: int = 0, n_self_pairs: int = 2000) -> Robot:
+    """14-DOF dual arm: two 7-DOF chains 0.6 m apart, each with two mimic finger joints."""
+    rng = np.random.default_rng(seed)
+    lengths, per = _arm_chain(rng, 56)
+    n_arm = 9  # 7 arm joints + 2 finger joints
+    twists = np.zeros((2 * n_arm, 6), np.float32)
+    parent_tf = np.zeros((2 * n_arm, 7), np.float32)
+    parent = np.zeros(2 * n_arm, np.int32)
+    act = np.full(2 * n_arm, -1, np.int32)
+    mimic = np.full(2 * n_arm, -1, np.int32)
+    links, local, radius, arm_of = [], [], [], []
+    for arm, y in enumerate((0.3, -0.3)):
+        b = arm * n_arm
+        for j, a in enumerate(AXES_7):
+            twists[b + j, 3:] = _AXIS[a]
+            parent_tf[b + j] = (1.0, 0.0, 0.0, 0.0, 0.0, y if j == 0 else 0.0, lengths[j])
+            parent[b + j] = -1 if j == 0 else b + j - 1
+            act[b + j] = arm * 7 + j
+        for f, fy in enumerate((0.03, -0.03)):
+            jf = b + 7 + f
+            twists[jf, 3:] = _AXIS["x"]
+            parent_tf[jf] = (1.0, 0.0, 0.0, 0.0, 0.0, fy, lengths[7])
+            parent[jf] = b + 6
+            mimic[jf] = arm * 7 + 6
+        l, lo, r = _arm_spheres(rng, per, lengths, 0.04, 0.08, link_base=b)
+        links += l; local += lo; radius += r; arm_of += [arm] * len(l)
+        for f in range(2):
+            zs = np.linspace(0.01, 0.07, 4)
+            for z in zs:
+                links.append(b + 7 + f)
+                local.append((0.0, 0.0, z))
+                radius.append(0.012)
+                arm_of.append(arm)
+    links = np.asarray(links, np.int32)
+    arm_of = np.asarray(arm_of)
+    # link index within its arm: 0..6 arm links, 7 for both fingers
+    rank = np.minimum(links % n_arm, 7)
+    cand = []
+    for i in range(len(links)):
+        for k in range(i + 1, len(links)):
+            if arm_of[i] != arm_of[k] or abs(int(rank[i]) - int(rank[k])) >= 2:
+                cand.append((i, k))
+    pairs = _pick_pairs(rng, cand, n_self_pairs)
+    ctx = {
+        "fk_twists": twists,
+        "fk_parent_tf": parent_tf,
+        "fk_parent_idx": parent,
+        "fk_act_idx": act,
+        "fk_mimic_mul": np.ones(2 * n_arm, np.float32),
+        "fk_mimic_off": np.zeros(2 * n_arm, np.float32),
+        "fk_mimic_act_idx": mimic,
+        "fk_topo_inv": np.arange(2 * n_arm, dtype=np.int32),
+        "sphere_link_idx": links,
+        "sphere_local": np.asarray(local, np.float32),
+        "sphere_radius": np.asarray(radius, np.float32),
+        "self_pairs": pairs.astype(np.int32),
+    }
+    return Robot(ctx, np.full(14, -LIMIT, np.float32), np.full(14, LIMIT, np.float32))
+
+
+def _random_quat(rng, n):
+    q = rng.normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def _quat_mat(q):
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def floor_for(robot: Robot) -> np.ndarray:
+    """Floor halfspace ``[n, p]`` (n = +z) just below the lowest base-link sphere."""
+    c = robot.ctx
+    root = int(np.flatnonzero(c["fk_parent_idx"] < 0)[0])
+    base_z = float(c["fk_parent_tf"][root, 6])
+    sel = c["sphere_link_idx"] == root
+    low = float(np.min(base_z + c["sphere_local"][sel, 2] - c["sphere_radius"][sel]))
+    return np.array([[0.0, 0.0, 1.0, 0.0, 0.0, low - 0.01]], np.float32)
+
+
+def make_world(rng, n_spheres, r_lo, r_hi, n_boxes=0, n_capsules=0, floor=None) -> dict:
+    spheres = np.zeros((n_spheres, 4), np.float32)
+    spheres[:, :3] = rng.uniform(REGION_LO, REGION_HI, size=(n_spheres, 3))
+    spheres[:, 3] = rng.uniform(r_lo, r_hi, size=n_spheres)
+    boxes = np.zeros((n_boxes, 15), np.float32)
+    if n_boxes:
+        centers = rng.uniform(REGION_LO, REGION_HI, size=(n_boxes, 3))
+        hl = rng.uniform(0.05, 0.3, size=(n_boxes, 3))
+        quats = _random_quat(rng, n_boxes)
+        for k in range(n_boxes):
+            R = _quat_mat(quats[k])
+            boxes[k] = np.concatenate([centers[k], R[:, 0], R[:, 1], R[:, 2], hl[k]])
+    caps = np.zeros((n_capsules, 7), np.float32)
+    if n_capsules:
+        centers = rng.uniform(REGION_LO, REGION_HI, size=(n_capsules, 3))
+        length = rng.uniform(0.2, 0.6, size=n_capsules)
+        rad = rng.uniform(0.03, 0.08, size=n_capsules)
+        d = rng.normal(size=(n_capsules, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        caps[:, 0:3] = centers - d * (length[:, None] / 2)
+        caps[:, 3:6] = centers + d * (length[:, None] / 2)
+        caps[:, 6] = rad
+    return {
+        "world_spheres": spheres,
+        "world_capsules": caps,
+        "world_boxes": boxes,
+        "world_halfspaces": floor if floor is not None else np.zeros((0, 6), np.float32),
+    }
+
+
+def scene_c1(robot_seed: int = 0, world_seed: int = 1) -> Scene:
+    robot = make_arm(robot_seed, 48, 60)
+    world = make_world(np.random.default_rng(world_seed), 40, 0.05, 0.15, floor=floor_for(robot))
+    return Scene(robot, world)
+
+
+def scene_c2(robot_seed: int = 0, world_seed: int = 1) -> Scene:
+    robot = make_arm(robot_seed, 64, 200)
+    world = make_world(np.random.default_rng(world_seed), 200, 0.02, 0.12, n_boxes=16, n_capsules=8,
+                       floor=floor_for(robot))
+    return Scene(robot, world)
+
+
+def scene_c4(robot_seed: int = 0, world_seed: int = 1) -> Scene:
+    robot = make_dual_arm(robot_seed)
+    world = make_world(np.random.default_rng(world_seed), 150, 0.02, 0.12, n_boxes=12, floor=floor_for(robot))
+    return Scene(robot, world)
+
+
+def scene_c5(robot_seed: int = 0, world_seed: int = 1) -> Scene:
+    robot = make_arm(robot_seed, 48, 60)
+    world = make_world(np.random.default_rng(world_seed), 100, 0.05, 0.15, floor=floor_for(robot))
+    return Scene(robot, world)
+
+
+C3_SPHERES = (10, 300)
+C3_BOXES = (0, 24)
+C3_GOALS = (1, 8)
+C3_NODES = (8192, 65536)
+
+
+def c3_query_world(robot: Robot, query: int, seed_base: int = 3000) -> tuple[dict, int, int]:
+    """World, goal count and max_nodes of query ``query`` of the 1024-query batch."""
+    rng = np.random.default_rng(seed_base + query)
+    ns = int(rng.integers(C3_SPHERES[0], C3_SPHERES[1] + 1))
+    nb = int(rng.integers(C3_BOXES[0], C3_BOXES[1] + 1))
+    ng = int(rng.integers(C3_GOALS[0], C3_GOALS[1] + 1))
+    world = make_world(rng, ns, 0.02, 0.12, n_boxes=nb, floor=floor_for(robot))
+    n_min = C3_SPHERES[0] + C3_BOXES[0]
+    n_max = C3_SPHERES[1] + C3_BOXES[1]
+    frac = (ns + nb - n_min) / (n_max - n_min)
+    nodes = C3_NODES[0] + (C3_NODES[1] - C3_NODES[0]) * frac
+    nodes = int(round(nodes / 64.0)) * 64
+    return world, ng, max(C3_NODES[0], min(C3_NODES[1], nodes))
+
+
+def sample_free(rng, robot: Robot, n: int, is_free, batch: int = 256, max_tries: int = 10_000_000):
+    """``n`` configurations drawn from U(limits) and kept when ``is_free`` accepts them."""
+    out = []
+    tried = 0
+    while len(out) < n:
+        if tried >= max_tries:
+            raise RuntimeError(f"sample_free: only {len(out)}/{n} free configs after {tried} draws")
+        q = rng.uniform(robot.lower, robot.upper, size=(batch, robot.dim)).astype(np.float32)
+        tried += batch
+        ok = np.asarray(is_free(q), dtype=bool)
+        for k in np.flatnonzero(ok):
+            out.append(q[k])
+            if len(out) == n:
+                break
+    return np.stack(out, axis=0)
+
+
+def query_seed(config: int, query: int) -> int:
+    """Seed of start/goal sampling for query ``query`` of workload ``config``."""
+    return 1000 * config + query

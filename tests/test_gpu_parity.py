@@ -1,0 +1,214 @@
+"""GPU parity: the HIP library against the CPU oracle, bit for bit.
+
+Every test calls the C ABI (through prrax_amd) on the device and compares with the oracle on
+the same seeded inputs.  Integer/index results and every float produced by the shared
+arithmetic (FK poses, tree configurations, distances) must be identical.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from prrax_amd import scenes
+from prrax_amd.prrtc import CollisionChecker, forward_kinematics, prrtc_nearest_neighbor, prrtc_plan, prrtc_plan_batch, prrtc_extend
+
+from helpers import compare_plan, query
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _robot_scenes():
+    return {"c1": scenes.scene_c1(), "c2": scenes.scene_c2(), "c4": scenes.scene_c4()}
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c4"])
+def test_fk_bit_exact(name):
+    sc = _robot_scenes()[name]
+    cc = sc.context()
+    dim = sc.robot.dim
+    rng = np.random.default_rng(7)
+    q = rng.uniform(-3.5, 3.5, size=(4096, dim)).astype(np.float32)
+    say(f"fk {name} n=4096")
+    got = forward_kinematics(cc, q).cpu().numpy()
+    ctx = O.Context(cc, dim)
+    want = np.stack([O.fk(ctx, qi) for qi in q[:512]])
+    np.testing.assert_array_equal(got[:512].view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("name,n", [("c1", 200_000), ("c2", 50_000), ("c4", 20_000)])
+@pytest.mark.parametrize("mode", ["binary", "differentiable"])
+def test_collision_bit_exact(name, n, mode):
+    sc = _robot_scenes()[name]
+    cc = sc.context()
+    dim = sc.robot.dim
+    rng = np.random.default_rng(11)
+    q = rng.uniform(sc.robot.lower, sc.robot.upper, size=(n, dim)).astype(np.float32)
+    say(f"collision {name} {mode} n={n}")
+    chk = CollisionChecker(cc, dim, collision_checker=mode, collision_margin=0.02)
+    hit = chk.in_collision(q).cpu().numpy()
+    mind = chk.min_distance(q[:5000]).cpu().numpy()
+    ctx = O.Context(cc, dim, collision_mode=1 if mode == "differentiable" else 0, collision_margin=0.02)
+    want = O.in_collision(ctx, q)
+    np.testing.assert_array_equal(hit, want)
+    np.testing.assert_array_equal(mind.view(np.uint32), O.min_distance(ctx, q[:5000]).view(np.uint32))
+    assert 0.05 < hit.mean() < 0.999  # both outcomes well represented
+
+
+def test_nearest_neighbor_ties_and_parity():
+    rng = np.random.default_rng(3)
+    for dim, n, nq in ((7, 1000, 1), (7, 16384, 32), (14, 4096, 512), (3, 77, 40)):
+        max_nodes = n + 13
+        tree = np.zeros((dim, max_nodes), np.float32)
+        tree[:, :n] = rng.uniform(-3, 3, size=(dim, n)).astype(np.float32)
+        dup = rng.choice(n, size=n // 10, replace=False)
+        tree[:, dup] = tree[:, dup[::-1]]  # duplicated nodes -> exact ties
+        qs = rng.uniform(-3, 3, size=(nq, dim)).astype(np.float32)
+        qs[: nq // 2] = tree[:, rng.choice(n, size=nq // 2)].T  # queries on nodes: distance 0 ties
+        say(f"nn dim={dim} n={n} nq={nq}")
+        dist, idx = prrtc_nearest_neighbor(tree, qs, n)
+        oi, od = O.nearest(tree, n, qs)
+        np.testing.assert_array_equal(idx.cpu().numpy(), oi)
+        np.testing.assert_array_equal(dist.cpu().numpy().view(np.uint32), np.sqrt(od).view(np.uint32))
+
+
+def test_extend_matches_steer():
+    rng = np.random.default_rng(5)
+    tree = rng.uniform(-3, 3, size=(7, 64)).astype(np.float32)
+    nn = rng.integers(0, 64, size=100).astype(np.int32)
+    smp = rng.uniform(-3, 3, size=(100, 7)).astype(np.float32)
+    smp[0] = tree[:, nn[0]]  # zero-distance guard
+    out, par, val = prrtc_extend(tree, nn, smp, 0.5)
+    out = out.cpu().numpy()
+    base = tree[:, nn].T
+    d = np.sqrt(np.sum((smp - base).astype(np.float64) ** 2, axis=1))
+    ref = base + (smp - base) * np.minimum(1.0, 0.5 / np.maximum(d, 1e-30))[:, None]
+    np.testing.assert_allclose(out, ref, atol=2e-6)
+    np.testing.assert_array_equal(par.cpu().numpy(), nn)
+    assert (val.cpu().numpy() == 1).all()
+
+
+C1_KW = dict(max_iterations=10_000, step_size=0.5, granularity=16, max_nodes=16384)
+
+
+@pytest.mark.parametrize("qi,S", [(0, 128), (1, 128), (2, 32), (3, 64), (4, 16)])
+def test_planner_c1_bit_exact(qi, S):
+    sc = scenes.scene_c1()
+    cc = sc.context()
+    start, goals = query(sc, 1, qi)
+    say(f"plan c1 q={qi} S={S}")
+    res = prrtc_plan(start, goals, num_new_samples=S, min_vals=sc.robot.lower, max_vals=sc.robot.upper,
+                     collision_context=cc, **C1_KW)
+    ora = O.plan(O.Context(cc, 7), start, goals, num_new_samples=S, min_vals=sc.robot.lower,
+                 max_vals=sc.robot.upper, **C1_KW)
+    compare_plan(res, ora, f"c1 q{qi}")
+    assert res.solved
+
+
+@pytest.mark.parametrize("kw", [
+    dict(balance_mode=0), dict(balance_mode=2, tree_ratio=1.0), dict(balance_mode=3), dict(dynamic_domain=False),
+    dict(granularity=64), dict(granularity=1), dict(step_size=0.2), dict(collision_checker="differentiable"),
+])
+def test_planner_variants_bit_exact(kw):
+    sc = scenes.scene_c1()
+    cc = sc.context()
+    start, goals = query(sc, 1, 10, n_goals=3)
+    base = dict(C1_KW, num_new_samples=48, min_vals=sc.robot.lower, max_vals=sc.robot.upper)
+    base.update(kw)
+    say(f"plan variant {kw}")
+    res = prrtc_plan(start, goals, collision_context=cc, **base)
+    mode = 1 if base.pop("collision_checker", "binary") == "differentiable" else 0
+    ora = O.plan(O.Context(cc, 7, collision_mode=mode, collision_margin=0.02), start, goals, **base)
+    compare_plan(res, ora, str(kw))
+
+
+def test_planner_failure_modes_bit_exact():
+    sc = scenes.scene_c1()
+    cc = sc.context()
+    start, goals = query(sc, 1, 20)
+    common = dict(num_new_samples=64, min_vals=sc.robot.lower, max_vals=sc.robot.upper)
+    # tree full
+    say("plan tree-full")
+    res = prrtc_plan(start, goals, collision_context=cc, max_nodes=40, max_iterations=100, **common)
+    ora = O.plan(O.Context(cc, 7), start, goals, max_nodes=40, max_iterations=100, **common)
+    compare_plan(res, ora, "tree-full")
+    assert res.status == O.STATUS_TREE_FULL and not res.solved
+    # iteration cap: an unreachable goal (in collision, isolated by a wall of spheres)
+    say("plan iteration-cap")
+    wall = dict(cc)
+    wall["world_spheres"] = np.concatenate([cc["world_spheres"], np.array([[0.0, 0.0, 0.0, 10.0]], np.float32)])
+    res = prrtc_plan(start, goals, collision_context=wall, max_nodes=4096, max_iterations=5, **common)
+    ora = O.plan(O.Context(wall, 7), start, goals, max_nodes=4096, max_iterations=5, **common)
+    compare_plan(res, ora, "iter-cap")
+    assert res.status == O.STATUS_ITER_CAP and res.iterations == 6
+
+
+def test_planner_no_collision_context():
+    say("plan geometric")
+    start = np.zeros(7, np.float32)
+    goals = np.full((1, 7), 2.0, np.float32)
+    res = prrtc_plan(start, goals, allow_unsafe_no_collision=True, max_nodes=4096, num_new_samples=32)
+    ctx = O.Context(scenes.make_arm().ctx | {"sphere_link_idx": np.zeros(0, np.int32),
+                                                "sphere_local": np.zeros((0, 3), np.float32),
+                                                "sphere_radius": np.zeros(0, np.float32)}, 7)
+    ora = O.plan(ctx, start, goals, max_nodes=4096, num_new_samples=32, max_iterations=1_000_000)
+    compare_plan(res, ora, "geometric")
+    assert res.solved
+
+
+def test_batch_uneven_matches_single_queries():
+    robot = scenes.make_arm(0, 64, 200)
+    B = 12
+    worlds, goals, starts, nodes, ctxs = [], [], [], [], []
+    for qi in range(B):
+        w, ng, N = scenes.c3_query_world(robot, qi)
+        cc = dict(robot.ctx)
+        cc.update(w)
+        ctxs.append(cc)
+        s, g = query(scenes.Scene(robot, w), 3, qi, n_goals=ng)
+        starts.append(s)
+        goals.append(g)
+        nodes.append(N)
+    kw = dict(num_new_samples=64, granularity=16, max_iterations=10_000, min_vals=robot.lower, max_vals=robot.upper)
+    say(f"batch uneven B={B}")
+    order = list(reversed(range(B)))
+    res = prrtc_plan_batch(np.stack(starts), goals, collision_context=ctxs, max_nodes=max(nodes),
+                           max_nodes_per_query=nodes, seeds=list(range(B)), order=order, **kw)
+    for qi in range(B):
+        ora = O.plan(O.Context(ctxs[qi], 7), starts[qi], goals[qi], max_nodes=nodes[qi], seed=qi, **kw)
+        compare_plan(res[qi], ora, f"batch q{qi}")
+    # the same queries one launch each give the same trees
+    for qi in (0, 5):
+        single = prrtc_plan(starts[qi], goals[qi], collision_context=ctxs[qi], max_nodes=nodes[qi], seed=qi, **kw)
+        ora = O.plan(O.Context(ctxs[qi], 7), starts[qi], goals[qi], max_nodes=nodes[qi], seed=qi, **kw)
+        compare_plan(single, ora, f"single q{qi}")
+
+
+def test_paths_valid_under_fp64_checker():
+    """Every returned path's edges are free at 4x the planner's waypoint resolution under the
+    FP64 checker (configs within 1e-5 m of contact are excluded as undecidable)."""
+    sc = scenes.scene_c2()
+    cc = sc.context()
+    starts, goals = [], []
+    for qi in range(6):
+        s, g = query(sc, 2, qi, n_goals=4)
+        starts.append(s)
+        goals.append(g)
+    say("plan c2 x6 batch")
+    res = prrtc_plan_batch(np.stack(starts), np.stack(goals), collision_context=cc, num_new_samples=512,
+                           max_nodes=65536, max_iterations=10_000, min_vals=sc.robot.lower, max_vals=sc.robot.upper)
+    for i, r in enumerate(res):
+        assert r.solved
+        p = r.path.astype(np.float64)
+        assert np.array_equal(r.path[0], starts[i])
+        assert any(np.array_equal(r.path[-1], g) for g in goals[i])
+        for a, b in zip(p[:-1], p[1:]):
+            for t in np.linspace(0.0, 1.0, 65):
+                d = O.min_distance64(cc, a + t * (b - a))
+                assert d > -1e-5, f"query {i}: path point in collision (d={d})"
