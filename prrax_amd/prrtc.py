@@ -375,6 +375,10 @@ def prepare_batch(
     if step_size <= 0.0:
         raise ValueError("step_size must be > 0")
     granularity = _check_granularity(granularity)
+    if isinstance(collision_context, (list, tuple)) and len(collision_context) != B:
+        raise ValueError(
+            "When collision_context is a list/tuple, it must have length "
+            f"equal to batch_size ({B}), got {len(collision_context)}")
     dev = _device()
     if isinstance(collision_context, (list, tuple)):
         if len(collision_context) != B:
