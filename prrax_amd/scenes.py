@@ -209,10 +209,54 @@ def floor_for(robot: Robot) -> np.ndarray:
     return np.array([[0.0, 0.0, 1.0, 0.0, 0.0, low - 0.01]], np.float32)
 
 
-def make_world(rng, n_spheres, r_lo, r_hi, n_boxes=0, n_capsules=0, floor=None) -> dict:
+def root_spheres(robot: Robot) -> np.ndarray:
+    """World ``[x, y, z, r]`` of the spheres on root frames whose twist is a rotation about the
+    frame's own z axis through the origin: those spheres sit on the axis, so they never move."""
+    c = robot.ctx
+    out = []
+    for j in np.flatnonzero(c["fk_parent_idx"] < 0):
+        tw = c["fk_twists"][j]
+        if np.any(tw[:3] != 0) or np.any(np.abs(tw[3:5]) > 0):
+            continue
+        R = _quat_mat(c["fk_parent_tf"][j, :4].astype(np.float64))
+        t = c["fk_parent_tf"][j, 4:].astype(np.float64)
+        for s in np.flatnonzero(c["sphere_link_idx"] == j):
+            loc = c["sphere_local"][s].astype(np.float64)
+            if abs(loc[0]) > 0 or abs(loc[1]) > 0:
+                continue
+            out.append(np.concatenate([R @ loc + t, [c["sphere_radius"][s]]]))
+    return np.asarray(out, np.float64).reshape(-1, 4)
+
+
+def _clear(kind: str, prim: np.ndarray, keep: np.ndarray, gap: float = 0.02) -> bool:
+    """True when the primitive is at least ``gap`` away from every sphere in ``keep``."""
+    if keep.shape[0] == 0:
+        return True
+    c, r = keep[:, :3], keep[:, 3]
+    if kind == "sphere":
+        d = np.linalg.norm(c - prim[:3], axis=1) - prim[3]
+    elif kind == "capsule":
+        a, b = prim[:3], prim[3:6]
+        ab = b - a
+        t = np.clip((c - a) @ ab / max(ab @ ab, 1e-300), 0.0, 1.0)
+        d = np.linalg.norm(c - (a + t[:, None] * ab), axis=1) - prim[6]
+    else:
+        R = np.stack([prim[3:6], prim[6:9], prim[9:12]], axis=1)
+        q = np.abs((c - prim[:3]) @ R) - prim[12:15]
+        d = np.linalg.norm(np.maximum(q, 0.0), axis=1) + np.minimum(q.max(axis=1), 0.0)
+    return bool(np.all(d - r >= gap))
+
+
+def make_world(rng, n_spheres, r_lo, r_hi, n_boxes=0, n_capsules=0, floor=None, keep_clear=None) -> dict:
+    """World primitives in the obstacle region; an obstacle that would touch a sphere of
+    ``keep_clear`` (the robot's immovable base) has its centre redrawn."""
+    keep = np.zeros((0, 4)) if keep_clear is None else np.asarray(keep_clear, np.float64)
     spheres = np.zeros((n_spheres, 4), np.float32)
     spheres[:, :3] = rng.uniform(REGION_LO, REGION_HI, size=(n_spheres, 3))
     spheres[:, 3] = rng.uniform(r_lo, r_hi, size=n_spheres)
+    for k in range(n_spheres):
+        while not _clear("sphere", spheres[k].astype(np.float64), keep):
+            spheres[k, :3] = rng.uniform(REGION_LO, REGION_HI)
     boxes = np.zeros((n_boxes, 15), np.float32)
     if n_boxes:
         centers = rng.uniform(REGION_LO, REGION_HI, size=(n_boxes, 3))
@@ -221,6 +265,8 @@ def make_world(rng, n_spheres, r_lo, r_hi, n_boxes=0, n_capsules=0, floor=None) 
         for k in range(n_boxes):
             R = _quat_mat(quats[k])
             boxes[k] = np.concatenate([centers[k], R[:, 0], R[:, 1], R[:, 2], hl[k]])
+            while not _clear("box", boxes[k].astype(np.float64), keep):
+                boxes[k, :3] = rng.uniform(REGION_LO, REGION_HI)
     caps = np.zeros((n_capsules, 7), np.float32)
     if n_capsules:
         centers = rng.uniform(REGION_LO, REGION_HI, size=(n_capsules, 3))
@@ -231,6 +277,11 @@ def make_world(rng, n_spheres, r_lo, r_hi, n_boxes=0, n_capsules=0, floor=None) 
         caps[:, 0:3] = centers - d * (length[:, None] / 2)
         caps[:, 3:6] = centers + d * (length[:, None] / 2)
         caps[:, 6] = rad
+        for k in range(n_capsules):
+            while not _clear("capsule", caps[k].astype(np.float64), keep):
+                c = rng.uniform(REGION_LO, REGION_HI)
+                caps[k, 0:3] = c - d[k] * length[k] / 2
+                caps[k, 3:6] = c + d[k] * length[k] / 2
     return {
         "world_spheres": spheres,
         "world_capsules": caps,
@@ -241,26 +292,26 @@ def make_world(rng, n_spheres, r_lo, r_hi, n_boxes=0, n_capsules=0, floor=None) 
 
 def scene_c1(robot_seed: int = 0, world_seed: int = 1) -> Scene:
     robot = make_arm(robot_seed, 48, 60)
-    world = make_world(np.random.default_rng(world_seed), 40, 0.05, 0.15, floor=floor_for(robot))
+    world = make_world(np.random.default_rng(world_seed), 40, 0.05, 0.15, floor=floor_for(robot), keep_clear=root_spheres(robot))
     return Scene(robot, world)
 
 
 def scene_c2(robot_seed: int = 0, world_seed: int = 1) -> Scene:
     robot = make_arm(robot_seed, 64, 200)
     world = make_world(np.random.default_rng(world_seed), 200, 0.02, 0.12, n_boxes=16, n_capsules=8,
-                       floor=floor_for(robot))
+                       floor=floor_for(robot), keep_clear=root_spheres(robot))
     return Scene(robot, world)
 
 
 def scene_c4(robot_seed: int = 0, world_seed: int = 1) -> Scene:
     robot = make_dual_arm(robot_seed)
-    world = make_world(np.random.default_rng(world_seed), 150, 0.02, 0.12, n_boxes=12, floor=floor_for(robot))
+    world = make_world(np.random.default_rng(world_seed), 150, 0.02, 0.12, n_boxes=12, floor=floor_for(robot), keep_clear=root_spheres(robot))
     return Scene(robot, world)
 
 
 def scene_c5(robot_seed: int = 0, world_seed: int = 1) -> Scene:
     robot = make_arm(robot_seed, 48, 60)
-    world = make_world(np.random.default_rng(world_seed), 100, 0.05, 0.15, floor=floor_for(robot))
+    world = make_world(np.random.default_rng(world_seed), 100, 0.05, 0.15, floor=floor_for(robot), keep_clear=root_spheres(robot))
     return Scene(robot, world)
 
 
@@ -276,7 +327,7 @@ def c3_query_world(robot: Robot, query: int, seed_base: int = 3000) -> tuple[dic
     ns = int(rng.integers(C3_SPHERES[0], C3_SPHERES[1] + 1))
     nb = int(rng.integers(C3_BOXES[0], C3_BOXES[1] + 1))
     ng = int(rng.integers(C3_GOALS[0], C3_GOALS[1] + 1))
-    world = make_world(rng, ns, 0.02, 0.12, n_boxes=nb, floor=floor_for(robot))
+    world = make_world(rng, ns, 0.02, 0.12, n_boxes=nb, floor=floor_for(robot), keep_clear=root_spheres(robot))
     n_min = C3_SPHERES[0] + C3_BOXES[0]
     n_max = C3_SPHERES[1] + C3_BOXES[1]
     frac = (ns + nb - n_min) / (n_max - n_min)
@@ -285,7 +336,7 @@ def c3_query_world(robot: Robot, query: int, seed_base: int = 3000) -> tuple[dic
     return world, ng, max(C3_NODES[0], min(C3_NODES[1], nodes))
 
 
-def sample_free(rng, robot: Robot, n: int, is_free, batch: int = 256, max_tries: int = 10_000_000):
+def sample_free(rng, robot: Robot, n: int, is_free, batch: int = 256, max_tries: int = 1_000_000):
     """``n`` configurations drawn from U(limits) and kept when ``is_free`` accepts them."""
     out = []
     tried = 0

