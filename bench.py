@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--per-query", type=int, default=-1, help="queries run as single launches (-1 = all)")
     ap.add_argument("--fkcc-configs", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--max-iterations", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -110,7 +111,7 @@ def main():
     t0 = time.perf_counter()
     robot, worlds, starts, goals, nodes = build_c3(args.queries, checker_factory)
     log(f"workload built in {time.perf_counter() - t0:.1f}s")
-    kw = dict(max_iterations=10_000, step_size=0.5, num_new_samples=args.samples, granularity=args.granularity,
+    kw = dict(max_iterations=args.max_iterations, step_size=0.5, num_new_samples=args.samples, granularity=args.granularity,
               min_vals=robot.lower, max_vals=robot.upper)
     # Longest queries first (more obstacles -> more work): results do not depend on the order.
     order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
@@ -118,9 +119,11 @@ def main():
     pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
                        max_nodes_per_query=nodes, order=order, threads=args.threads, slots=args.slots, **kw)
     # ---- batched
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
+        t = time.perf_counter()
         pb.enqueue()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        log(f"warmup step {w}: {time.perf_counter() - t:.3f}s")
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record()
@@ -144,11 +147,13 @@ def main():
     singles[0].enqueue()
     torch.cuda.synchronize()
     times = []
-    for sp in singles:
+    for k, sp in enumerate(singles):
         t = time.perf_counter()
         sp.enqueue()
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
+        if (k + 1) % 128 == 0:
+            log(f"per-query launches: {k + 1}/{n_pq} done, {sum(times):.2f}s")
     solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
     pq_plans_per_s = n_pq / sum(times)
     same = bool(np.array_equal(solved_pq, solved_b[:n_pq]))
@@ -197,7 +202,7 @@ def main():
         "config": {"workload": f"{args.queries} uneven 7-DOF queries (10-300 spheres, 0-24 boxes, floor, 1-8 goals, "
                                "max_nodes 8192-65536), one persistent launch",
                    "queries": args.queries, "num_new_samples": args.samples, "granularity": args.granularity,
-                   "step_size": 0.5, "max_iterations": 10_000, "threads": args.threads, "slots": args.slots},
+                   "step_size": 0.5, "max_iterations": args.max_iterations, "threads": args.threads, "slots": args.slots},
         "batched": {"plans_per_s": round(plans_per_s, 2), "solve_rate": round(float(solved_b.mean()), 4),
                     "p50_ms_per_plan_device": round(float(np.median(summ["kernel_time_ms"])), 3),
                     "kernel_ms_per_step": round(kernel_ms, 3), "configs_checked": stats["configs"],
