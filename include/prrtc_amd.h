@@ -114,7 +114,7 @@ typedef struct {
 
 /* Launch shape (NULL = defaults chosen by the library). */
 typedef struct {
-    int32_t threads;  /* threads per work-group, multiple of 64, <= 1024; 0 = auto */
+    int32_t threads;  /* threads per work-group, multiple of 64 (planner <= 512, checks <= 1024); 0 = auto */
     int32_t slots;    /* resident work-groups (queries in flight); 0 = auto */
 } prrtc_launch;
 

@@ -35,13 +35,16 @@ struct WorldDev {
     const int* counts;                             // [B,5] or null
 };
 
-// Byte offsets of the model inside dynamic LDS, followed by per-lane scratch.
+// Byte offsets of the model inside dynamic LDS, followed by the per-wave FK areas: wave w owns
+// wq + w * kMaxDim * fbp floats (configurations, element c of dim d at [d * fbp + c]) and
+// wT + w * nj * 7 * fbp floats (joint poses, element c of slot k at [k * fbp + c]).
 struct ModelLayout {
-    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, lane_q, lane_T, end;
+    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, wq, wT, end;
+    int fbp;
 };
 
 __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, int nws, int nwc, int nwb, int nwh,
-                                                      int nt) {
+                                                      int nwaves, int fbp) {
     ModelLayout L;
     size_t o = 0;
     L.joints = o; o = align16(o + sizeof(JointRec) * nj);
@@ -52,9 +55,10 @@ __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, in
     L.wcap = o; o = align16(o + sizeof(CapsuleL) * nwc);
     L.wbox = o; o = align16(o + sizeof(BoxL) * nwb);
     L.whs = o; o = align16(o + sizeof(float4) * nwh);
-    L.lane_q = o; o = align16(o + sizeof(float) * kMaxDim * nt);
-    L.lane_T = o; o = align16(o + sizeof(float) * 7 * nj * nt);
+    L.wq = o; o = align16(o + sizeof(float) * kMaxDim * fbp * nwaves);
+    L.wT = o; o = align16(o + sizeof(float) * 7 * nj * fbp * nwaves);
     L.end = o;
+    L.fbp = fbp;
     return L;
 }
 
@@ -214,6 +218,28 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
     M.margin = margin;
     __syncthreads();
 }
+
+// Per-wave FK area of the calling wave.
+struct WaveArea {
+    float* q;
+    float* T;
+};
+
+__device__ __forceinline__ WaveArea wave_area(char* lds, const ModelLayout& Lo, int nj) {
+    const int w = threadIdx.x / kWave;
+    WaveArea a;
+    a.q = reinterpret_cast<float*>(lds + Lo.wq) + w * kMaxDim * Lo.fbp;
+    a.T = reinterpret_cast<float*>(lds + Lo.wT) + w * nj * 7 * Lo.fbp;
+    return a;
+}
+
+// Orders this wave's LDS writes before its later LDS reads from other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // ---------------------------------------------------------------------------------- nearest neighbour
 constexpr int kEC = 16;  // queries per tree pass

@@ -296,136 +296,134 @@ __device__ __forceinline__ void fk_lane(const Model& M, const LaneScratch& L) {
     }
 }
 
-// Loads joint js's transform as a rotation matrix + translation.
-__device__ __forceinline__ void load_frame(const LaneScratch& L, int js, Mat3& R, float& tx, float& ty, float& tz) {
-    const int nt = L.nt;
-    const float* T = L.T + (js * 7) * nt;
-    const Quat q = {T[0], T[nt], T[2 * nt], T[3 * nt]};
+// ---------------------------------------------------------------------------------- wave checks
+// One wave checks one configuration: lane l takes robot spheres l, l+64, ...; every lane walks the
+// same obstacle list (LDS broadcast), and the wave leaves as soon as any lane reports a contact
+// (ballot), so the exit is wave-uniform.  The verdicts are an OR (binary) or a min (margin) over
+// all tests, so the order of the tests does not change the result.
+//
+// The configuration's joint frames come from a per-wave FK area T (element c of slot k at
+// T[k * ts]); fk_lane fills it for a batch of configurations, one per lane.
+//
+// flops accumulates algorithmic flops of the tests performed (FMA = 2): 33 per sphere transform,
+// 10 / 27 / 36 / 9 per sphere-sphere / capsule / box / halfspace test, 10 per self pair.
+constexpr int kFB = 16;        // configurations per FK batch
+constexpr int kFBP = kFB + 1;  // padded stride of the per-wave FK area (spreads LDS banks)
+constexpr unsigned kFlopsXf = 33, kFlopsSS = 10, kFlopsSC = 27, kFlopsSB = 36, kFlopsSH = 9, kFlopsPair = 10;
+constexpr unsigned kFlopsJoint = 128;
+
+__device__ __forceinline__ void frame_from(const float* T, int ts, int js, Mat3& R, float& tx, float& ty, float& tz) {
+    const float* F = T + (js * 7) * ts;
+    const Quat q = {F[0], F[ts], F[2 * ts], F[3 * ts]};
     R = quat_mat(q);
-    tx = T[4 * nt]; ty = T[5 * nt]; tz = T[6 * nt];
+    tx = F[4 * ts]; ty = F[5 * ts]; tz = F[6 * ts];
 }
 
-// ---------------------------------------------------------------------------------- predicates
-// Binary world test of one robot sphere (early exit on the first hit).
-__device__ __forceinline__ bool sphere_hits_world(const Model& M, float sx, float sy, float sz, float sr) {
-    int i = 0;
-    for (; i + 4 <= M.n_ws; i += 4) {
-        bool h = hit_sphere(sx, sy, sz, sr, M.wsph[i]);
-        h |= hit_sphere(sx, sy, sz, sr, M.wsph[i + 1]);
-        h |= hit_sphere(sx, sy, sz, sr, M.wsph[i + 2]);
-        h |= hit_sphere(sx, sy, sz, sr, M.wsph[i + 3]);
-        if (h) return true;
-    }
-    for (; i < M.n_ws; ++i)
-        if (hit_sphere(sx, sy, sz, sr, M.wsph[i])) return true;
-    for (int k = 0; k < M.n_wc; ++k)
-        if (hit_capsule(sx, sy, sz, sr, M.wcap[k])) return true;
-    for (int k = 0; k < M.n_wb; ++k)
-        if (hit_box(sx, sy, sz, sr, M.wbox[k])) return true;
-    for (int k = 0; k < M.n_wh; ++k)
-        if (hit_halfspace(sx, sy, sz, sr, M.whs[k])) return true;
-    return false;
-}
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
-__device__ __forceinline__ float sphere_world_min(const Model& M, float sx, float sy, float sz, float sr, float best) {
-    for (int i = 0; i < M.n_ws; ++i) best = fminf(best, dist_sphere(sx, sy, sz, sr, M.wsph[i]));
-    for (int k = 0; k < M.n_wc; ++k) best = fminf(best, dist_capsule(sx, sy, sz, sr, M.wcap[k]));
-    for (int k = 0; k < M.n_wb; ++k) best = fminf(best, dist_box(sx, sy, sz, sr, M.wbox[k]));
-    for (int k = 0; k < M.n_wh; ++k) best = fminf(best, dist_halfspace(sx, sy, sz, sr, M.whs[k]));
-    return best;
-}
-
-// Work actually done by one predicate call: robot spheres tested against the world and self
-// pairs tested (an early exit counts the sphere it stopped in as fully tested).
-struct Work {
-    unsigned spheres = 0, pairs = 0;
-};
-
-// Binary predicate (prrtc_planner.cu:232-307): any world or self contact.  The config is in
-// L.q; FK goes to L.T.
-__device__ __forceinline__ bool lane_hits(const Model& M, const LaneScratch& L, Work& w) {
-    if (!M.enabled) return false;
-    fk_lane(M, L);
-    int cur = -1;
-    Mat3 R;
-    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-    for (int s = 0; s < M.n_spheres; ++s) {
-        const int js = M.sphere_js[s];
-        if (js < 0) continue;
-        if (js != cur) { load_frame(L, js, R, tx, ty, tz); cur = js; }
-        const float4 sp = M.spheres[s];
-        const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
-        ++w.spheres;
-        if (sphere_hits_world(M, p.x, p.y, p.z, sp.w)) return true;
-    }
-    int ci = -1, cj = -1;
-    Mat3 Ri, Rj;
-    float ix = 0.0f, iy = 0.0f, iz = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
-    for (int k = 0; k < M.n_pairs; ++k) {
-        const int2 pr = M.pairs[k];
-        const int ji = M.sphere_js[pr.x], jj = M.sphere_js[pr.y];
-        if (ji != ci) { load_frame(L, ji, Ri, ix, iy, iz); ci = ji; }
-        if (jj != cj) { load_frame(L, jj, Rj, jx, jy, jz); cj = jj; }
-        const float4 a = M.spheres[pr.x];
-        const float4 b = M.spheres[pr.y];
-        const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
-        const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
-        ++w.pairs;
-        if (hit_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w))) return true;
-    }
-    return false;
-}
-
-// Minimum signed distance over every valid pair (FLT_MAX when none), no early exit.
-__device__ __forceinline__ float lane_min_distance(const Model& M, const LaneScratch& L, Work& w) {
-    fk_lane(M, L);
+template <bool MARGIN>
+__device__ __forceinline__ float wave_check(const Model& M, const float* T, int ts, unsigned long long& flops) {
+    const int lane = threadIdx.x & (kWave - 1);
     float best = FLT_MAX;
-    int cur = -1;
-    Mat3 R;
-    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-    for (int s = 0; s < M.n_spheres; ++s) {
-        const int js = M.sphere_js[s];
-        if (js < 0) continue;
-        if (js != cur) { load_frame(L, js, R, tx, ty, tz); cur = js; }
-        const float4 sp = M.spheres[s];
-        const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
-        ++w.spheres;
-        best = sphere_world_min(M, p.x, p.y, p.z, sp.w, best);
+    for (int s0 = 0; s0 < M.n_spheres; s0 += kWave) {
+        const int s = s0 + lane;
+        int js = -1;
+        float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (s < M.n_spheres) { js = M.sphere_js[s]; sp = M.spheres[s]; }
+        const bool act = js >= 0;
+        const unsigned na = __popcll(__ballot(act));
+        if (na == 0) continue;
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+        if (act) {
+            Mat3 R;
+            float tx, ty, tz;
+            frame_from(T, ts, js, R, tx, ty, tz);
+            const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+            px = p.x; py = p.y; pz = p.z;
+        }
+        flops += na * kFlopsXf;
+        float cb = FLT_MAX;  // this chunk's minimum (margin mode)
+        for (int i = 0; i < M.n_ws; i += 8) {
+            const int ne = min(8, M.n_ws - i);
+            bool h = false;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k < ne) {
+                    if (MARGIN) cb = fminf(cb, dist_sphere(px, py, pz, sp.w, M.wsph[i + k]));
+                    else h |= hit_sphere(px, py, pz, sp.w, M.wsph[i + k]);
+                }
+            }
+            flops += static_cast<unsigned long long>(na) * ne * kFlopsSS;
+            if (!MARGIN && wave_any(act && h)) return 1.0f;
+        }
+        for (int i = 0; i < M.n_wc; i += 4) {
+            const int ne = min(4, M.n_wc - i);
+            bool h = false;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < ne) {
+                    if (MARGIN) cb = fminf(cb, dist_capsule(px, py, pz, sp.w, M.wcap[i + k]));
+                    else h |= hit_capsule(px, py, pz, sp.w, M.wcap[i + k]);
+                }
+            }
+            flops += static_cast<unsigned long long>(na) * ne * kFlopsSC;
+            if (!MARGIN && wave_any(act && h)) return 1.0f;
+        }
+        for (int i = 0; i < M.n_wb; i += 4) {
+            const int ne = min(4, M.n_wb - i);
+            bool h = false;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < ne) {
+                    if (MARGIN) cb = fminf(cb, dist_box(px, py, pz, sp.w, M.wbox[i + k]));
+                    else h |= hit_box(px, py, pz, sp.w, M.wbox[i + k]);
+                }
+            }
+            flops += static_cast<unsigned long long>(na) * ne * kFlopsSB;
+            if (!MARGIN && wave_any(act && h)) return 1.0f;
+        }
+        {
+            bool h = false;
+            for (int k = 0; k < M.n_wh; ++k) {
+                if (MARGIN) cb = fminf(cb, dist_halfspace(px, py, pz, sp.w, M.whs[k]));
+                else h |= hit_halfspace(px, py, pz, sp.w, M.whs[k]);
+            }
+            flops += static_cast<unsigned long long>(na) * M.n_wh * kFlopsSH;
+            if (!MARGIN && wave_any(act && h)) return 1.0f;
+        }
+        if (MARGIN && act) best = fminf(best, cb);
     }
-    int ci = -1, cj = -1;
-    Mat3 Ri, Rj;
-    float ix = 0.0f, iy = 0.0f, iz = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
-    for (int k = 0; k < M.n_pairs; ++k) {
-        const int2 pr = M.pairs[k];
-        const int ji = M.sphere_js[pr.x], jj = M.sphere_js[pr.y];
-        if (ji != ci) { load_frame(L, ji, Ri, ix, iy, iz); ci = ji; }
-        if (jj != cj) { load_frame(L, jj, Rj, jx, jy, jz); cj = jj; }
-        const float4 a = M.spheres[pr.x];
-        const float4 b = M.spheres[pr.y];
-        const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
-        const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
-        ++w.pairs;
-        best = fminf(best, dist_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w)));
+    for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
+        const int k = p0 + lane;
+        bool h = false;
+        if (k < M.n_pairs) {
+            const int2 pr = M.pairs[k];
+            const float4 a = M.spheres[pr.x];
+            const float4 b = M.spheres[pr.y];
+            Mat3 Ri, Rj;
+            float ix, iy, iz, jx, jy, jz;
+            frame_from(T, ts, M.sphere_js[pr.x], Ri, ix, iy, iz);
+            frame_from(T, ts, M.sphere_js[pr.y], Rj, jx, jy, jz);
+            const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
+            const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
+            if (MARGIN) best = fminf(best, dist_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w)));
+            else h = hit_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w));
+        }
+        flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
+        if (!MARGIN && wave_any(h)) return 1.0f;
     }
+    if (!MARGIN) return 0.0f;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) best = fminf(best, __shfl_xor(best, o, kWave));
     return best;
 }
 
-// The planner's collision verdict for the config in L.q (prrtc_planner.cu:501-531).
-__device__ __forceinline__ bool lane_in_collision(const Model& M, const LaneScratch& L, Work& w) {
+// The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform.
+__device__ __forceinline__ bool wave_in_collision(const Model& M, const float* T, int ts, unsigned long long& flops) {
     if (!M.enabled) return false;
-    if (M.mode == 1) return lane_min_distance(M, L, w) < M.margin;
-    return lane_hits(M, L, w);
-}
-
-// Algorithmic flops of the work counted in w for one query's model (FMA = 2): FK 128 per
-// revolute joint, 33 per sphere transform, and per sphere-primitive test 10 (sphere), 27
-// (capsule), 36 (box), 9 (halfspace); 10 per self pair.
-__device__ __forceinline__ double work_flops(const Model& M, unsigned long long configs, unsigned long long spheres,
-                                             unsigned long long pairs) {
-    const double world = 10.0 * M.n_ws + 27.0 * M.n_wc + 36.0 * M.n_wb + 9.0 * M.n_wh;
-    const double fk = M.enabled ? 128.0 * M.n_joints : 0.0;
-    return fk * static_cast<double>(configs) + (33.0 + world) * static_cast<double>(spheres) +
-           10.0 * static_cast<double>(pairs);
+    flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
+    if (M.mode == 1) return wave_check<true>(M, T, ts, flops) < M.margin;
+    return wave_check<false>(M, T, ts, flops) != 0.0f;
 }
 
 }  // namespace prrtc

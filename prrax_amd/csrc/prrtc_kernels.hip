@@ -28,30 +28,49 @@ struct CheckArgs {
     unsigned long long* flops;  // optional: algorithmic flops of the work done
 };
 
+// Waves take batches of kFB configurations: FK one per lane into the wave's FK area, then each
+// configuration is checked by the whole wave (lanes over robot spheres).
 __global__ __launch_bounds__(1024) void check_kernel(CheckArgs a) {
     extern __shared__ __align__(16) char lds[];
     int* scan = reinterpret_cast<int*>(lds + a.off_scan);
     Model M;
     stage_model(M, lds, a.ml, a.robot, a.world, 0, a.dim, a.mode, a.margin, scan);
+    const int lane = threadIdx.x & (kWave - 1);
+    const WaveArea W = wave_area(lds, a.ml, M.n_joints);
     LaneScratch L;
-    L.q = reinterpret_cast<float*>(lds + a.ml.lane_q) + threadIdx.x;
-    L.T = reinterpret_cast<float*>(lds + a.ml.lane_T) + threadIdx.x;
-    L.nt = blockDim.x;
-    const int stride = gridDim.x * blockDim.x;
-    Work w;
-    unsigned long long configs = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        for (int d = 0; d < a.dim; ++d) L.q[d * L.nt] = a.configs[static_cast<size_t>(i) * a.dim + d];
-        ++configs;
-        if (a.out_min) {
-            const float md = lane_min_distance(M, L, w);
-            a.out_min[i] = md;
-            if (a.out_hit) a.out_hit[i] = (a.mode == 1) ? (M.enabled && md < a.margin) : lane_in_collision(M, L, w);
-        } else {
-            a.out_hit[i] = lane_in_collision(M, L, w) ? 1 : 0;
+    L.q = W.q + lane;
+    L.T = W.T + lane;
+    L.nt = kFBP;
+    const int wpb = blockDim.x / kWave;
+    const int total_waves = gridDim.x * wpb;
+    unsigned long long flops = 0;
+    const int nb = (a.n + kFB - 1) / kFB;
+    for (int b = blockIdx.x * wpb + static_cast<int>(threadIdx.x) / kWave; b < nb; b += total_waves) {
+        const int i = b * kFB + lane;
+        if (lane < kFB && i < a.n) {
+            for (int d = 0; d < a.dim; ++d) L.q[d * kFBP] = a.configs[static_cast<size_t>(i) * a.dim + d];
+            if (M.n_joints > 0) fk_lane(M, L);
         }
+        wave_sync();
+        const int nc = min(kFB, a.n - b * kFB);
+        for (int c = 0; c < nc; ++c) {
+            const int idx = b * kFB + c;
+            if (a.out_min) {
+                flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
+                const float md = wave_check<true>(M, W.T + c, kFBP, flops);
+                if (lane == 0) a.out_min[idx] = md;
+                if (a.out_hit) {
+                    const bool h = (a.mode == 1) ? (M.enabled && md < a.margin) : wave_in_collision(M, W.T + c, kFBP, flops);
+                    if (lane == 0) a.out_hit[idx] = h ? 1 : 0;
+                }
+            } else {
+                const bool h = wave_in_collision(M, W.T + c, kFBP, flops);
+                if (lane == 0) a.out_hit[idx] = h ? 1 : 0;
+            }
+        }
+        wave_sync();
     }
-    if (a.flops && configs) global_add(a.flops, static_cast<unsigned long long>(work_flops(M, configs, w.spheres, w.pairs)));
+    if (lane == 0 && a.flops && flops) global_add(a.flops, flops);
 }
 
 struct FkArgs {
@@ -70,11 +89,11 @@ __global__ __launch_bounds__(256) void fk_kernel(FkArgs a) {
     Model M;
     stage_model(M, lds, a.ml, a.robot, a.world, 0, a.dim, 0, 0.0f, scan);
     LaneScratch L;
-    L.q = reinterpret_cast<float*>(lds + a.ml.lane_q) + threadIdx.x;
-    L.T = reinterpret_cast<float*>(lds + a.ml.lane_T) + threadIdx.x;
-    L.nt = blockDim.x;
     const int nj = M.n_joints;
     const int stride = gridDim.x * blockDim.x;
+    L.q = reinterpret_cast<float*>(lds + a.ml.wq) + threadIdx.x;
+    L.T = reinterpret_cast<float*>(lds + a.ml.wT) + threadIdx.x;
+    L.nt = blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         for (int d = 0; d < a.dim; ++d) L.q[d * L.nt] = a.configs[static_cast<size_t>(i) * a.dim + d];
         fk_lane(M, L);
@@ -142,10 +161,12 @@ __global__ void extend_kernel(const float* tree, int dim, int ncap, const int* n
 
 size_t scan_offset(const ModelLayout& ml) { return ml.end; }
 
-int model_lds(const prrtc_robot& r, const prrtc_world& w, int& nt, int max_lds, ModelLayout& ml, size_t& lds) {
+// per_lane: one FK slot per thread (FK kernel); otherwise per-wave FK batches (check kernel).
+int model_lds(const prrtc_robot& r, const prrtc_world& w, int& nt, int max_lds, ModelLayout& ml, size_t& lds,
+              bool per_lane) {
     for (;;) {
         ml = model_layout(r.n_joints, r.n_robot_spheres, w.n_self_pairs, w.n_world_spheres, w.n_world_capsules,
-                          w.n_world_boxes, w.n_world_halfspaces, nt);
+                          w.n_world_boxes, w.n_world_halfspaces, per_lane ? 1 : nt / kWave, per_lane ? nt : kFBP);
         lds = align16(ml.end + sizeof(int) * 20);
         if (lds <= static_cast<size_t>(max_lds) || nt <= kWave) break;
         nt -= kWave;
@@ -180,7 +201,7 @@ int prrtc_check_configs(prrtc_stream_t stream, const float* configs, int32_t n, 
     CheckArgs a;
     std::memset(&a, 0, sizeof(a));
     size_t lds = 0;
-    rc = model_lds(*robot, *world, nt, di.max_lds, a.ml, lds);
+    rc = model_lds(*robot, *world, nt, di.max_lds, a.ml, lds, false);
     if (rc) return rc;
     a.robot = robot_dev(*robot);
     a.world = world_dev(*world, false);
@@ -188,8 +209,9 @@ int prrtc_check_configs(prrtc_stream_t stream, const float* configs, int32_t n, 
     a.configs = configs; a.n = n; a.dim = dim; a.mode = (collision_mode == 1) ? 1 : 0; a.margin = collision_margin;
     a.out_hit = out_hit; a.out_min = out_min_dist;
     a.flops = reinterpret_cast<unsigned long long*>(work_flops);
-    int grid = (launch && launch->slots > 0) ? launch->slots : (n + nt - 1) / nt;
-    grid = std::max(1, std::min(grid, 65535));
+    const int batches = (n + kFB - 1) / kFB;
+    int grid = (launch && launch->slots > 0) ? launch->slots : (batches + nt / kWave - 1) / (nt / kWave);
+    grid = std::max(1, std::min(grid, 8 * di.cus));
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&check_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
@@ -217,7 +239,7 @@ int prrtc_forward_kinematics(prrtc_stream_t stream, const float* configs, int32_
     FkArgs a;
     std::memset(&a, 0, sizeof(a));
     size_t lds = 0;
-    rc = model_lds(rb, empty, nt, di.max_lds, a.ml, lds);
+    rc = model_lds(rb, empty, nt, di.max_lds, a.ml, lds, true);
     if (rc) return rc;
     if (nt != 256) return fail_with(PRRTC_E_LDS, "robot too large for the FK kernel");
     a.robot = robot_dev(rb);

@@ -190,10 +190,12 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
     const unsigned long long t0 = wall_clock64();
     Model M;
     stage_model(M, lds, a.ml, a.robot, a.world, q, a.dim, a.mode, a.margin, scan);
-    LaneScratch L;
-    L.q = reinterpret_cast<float*>(lds + a.ml.lane_q) + tid;
-    L.T = reinterpret_cast<float*>(lds + a.ml.lane_T) + tid;
-    L.nt = nt;
+    const int lane = tid & (kWave - 1), wid = tid / kWave, nwaves = nt / kWave;
+    const WaveArea W = wave_area(lds, a.ml, M.n_joints);
+    LaneScratch L;  // FK slot of this lane (lanes < kFB) in the wave's FK area
+    L.q = W.q + lane;
+    L.T = W.T + lane;
+    L.nt = kFBP;
 
     float* TA = a.tree_a + static_cast<size_t>(q) * dim * ncap;
     float* TB = a.tree_b + static_cast<size_t>(q) * dim * ncap;
@@ -222,8 +224,7 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
     }
     __syncthreads();
 
-    unsigned long long evals = 0;
-    Work work;
+    unsigned long long evals = 0, flops = 0;  // lane 0 of each wave counts
     int r = 0;
     for (r = 1;; ++r) {
         if (r > a.max_iterations) {
@@ -289,22 +290,38 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
             fail[e] = 0;
         }
         __syncthreads();
-        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1
+        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Each wave takes kFB
+        // (explorer, waypoint) items, computes their FK one per lane, then checks them one by one.
         {
             const int nv = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
             const int items = nv * G;
-            for (int it = tid; it < items; it += nt) {
-                const int slot = it / G, k = it - slot * G;
-                const int e = list[slot];
-                if (lds_load(&fail[e])) continue;
-                const float* T = side[e] ? TB : TA;
-                const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
-                for (int d = 0; d < dim; ++d) {
-                    const float base = T[static_cast<size_t>(d) * ncap + nn[e]];
-                    L.q[d * nt] = base + tf * (cand[e * dim + d] - base);
+            const int nb = (items + kFB - 1) / kFB;
+            for (int b = wid; b < nb; b += nwaves) {
+                const int it = b * kFB + lane;
+                int e = -1;
+                if (lane < kFB && it < items) {
+                    const int slot = it / G, k = it - slot * G;
+                    e = list[slot];
+                    if (lds_load(&fail[e])) {
+                        e = -1;
+                    } else {
+                        const float* T = side[e] ? TB : TA;
+                        const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
+                        for (int d = 0; d < dim; ++d) {
+                            const float base = T[static_cast<size_t>(d) * ncap + nn[e]];
+                            L.q[d * kFBP] = base + tf * (cand[e * dim + d] - base);
+                        }
+                        if (M.enabled) fk_lane(M, L);
+                    }
                 }
-                ++evals;
-                if (lane_in_collision(M, L, work)) lds_store(&fail[e], 1);
+                wave_sync();
+                for (int c = 0; c < kFB; ++c) {
+                    const int ec = uniform(__shfl(e, c, kWave));
+                    if (ec < 0 || uniform(lds_load(&fail[ec]))) continue;
+                    if (lane == 0) ++evals;
+                    if (wave_in_collision(M, W.T + c, kFBP, flops) && lane == 0) lds_store(&fail[ec], 1);
+                }
+                wave_sync();
             }
             __syncthreads();
         }
@@ -395,7 +412,7 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
             const int na = compact(S, [&](int e) { return newi[e] >= 0 && fail[e] == next[e] && kd[e] < next[e]; },
                                    list, scan);
             if (na == 0) break;
-            const int mper = max(1, (nt * kPassMult) / (na * G));
+            const int mper = max(1, (nwaves * kFB * kPassMult) / (na * G));
             int run = 0;
             for (int s0 = 0; s0 < na; s0 += nt) {
                 const int s = s0 + tid;
@@ -411,25 +428,42 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
             }
             if (tid == 0) offs[na] = run;
             __syncthreads();
-            for (int it = tid; it < run; it += nt) {
-                int lo = 0, hi = na;  // offs[lo] <= it < offs[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (offs[mid] <= it) lo = mid; else hi = mid;
+            const int nb = (run + kFB - 1) / kFB;
+            for (int b = wid; b < nb; b += nwaves) {
+                const int it = b * kFB + lane;
+                int e = -1, j = 0;
+                if (lane < kFB && it < run) {
+                    int lo = 0, hi = na;  // offs[lo] <= it < offs[hi]
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (offs[mid] <= it) lo = mid; else hi = mid;
+                    }
+                    e = list[lo];
+                    const int local = it - offs[lo];
+                    j = kd[e] + local / G;
+                    const int k = local - (local / G) * G;
+                    if (j >= lds_load(&fail[e])) {
+                        e = -1;
+                    } else {
+                        const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
+                        for (int d = 0; d < dim; ++d) {
+                            float c = cand[e * dim + d];
+                            const float v = vec[e * dim + d];
+                            for (int jj = 0; jj < j; ++jj) c += v;
+                            L.q[d * kFBP] = c + tf * v;
+                        }
+                        if (M.enabled) fk_lane(M, L);
+                    }
                 }
-                const int e = list[lo];
-                const int local = it - offs[lo];
-                const int j = kd[e] + local / G, k = local - (local / G) * G;
-                if (j >= lds_load(&fail[e])) continue;
-                const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
-                for (int d = 0; d < dim; ++d) {
-                    float c = cand[e * dim + d];
-                    const float v = vec[e * dim + d];
-                    for (int jj = 0; jj < j; ++jj) c += v;
-                    L.q[d * nt] = c + tf * v;
+                wave_sync();
+                for (int c = 0; c < kFB; ++c) {
+                    const int ec = uniform(__shfl(e, c, kWave));
+                    const int jc = uniform(__shfl(j, c, kWave));
+                    if (ec < 0 || jc >= uniform(lds_load(&fail[ec]))) continue;
+                    if (lane == 0) ++evals;
+                    if (wave_in_collision(M, W.T + c, kFBP, flops) && lane == 0) lds_min(&fail[ec], jc);
                 }
-                ++evals;
-                if (lane_in_collision(M, L, work)) lds_min(&fail[e], j);
+                wave_sync();
             }
             __syncthreads();
             for (int s = tid; s < na; s += nt) {
@@ -499,9 +533,9 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
         }
     }
     // ---- outputs
-    if (evals) {
+    if (lane == 0 && evals) {
         global_add(&a.stats[0], evals);
-        global_add(&a.stats[2], static_cast<unsigned long long>(work_flops(M, evals, work.spheres, work.pairs)));
+        global_add(&a.stats[2], flops);
     }
     const int st = sc[SC_STATUS];
     if (tid == 0) {
@@ -560,7 +594,7 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
 }
 
 template <int MAXD>
-__global__ __launch_bounds__(1024) void plan_kernel(PlanArgs a) {
+__global__ __launch_bounds__(512) void plan_kernel(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
     int* sc = reinterpret_cast<int*>(lds + a.off_misc) + 20;
     for (;;) {
@@ -727,7 +761,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     if (hipGetDevice(&dev) != hipSuccess) return fail_with(PRRTC_E_HIP, "hipGetDevice failed");
     const DevInfo& di = dev_info(dev);
     int nt = (launch && launch->threads > 0) ? launch->threads : 256;
-    if (nt % kWave || nt > 1024) return fail_with(PRRTC_E_INVALID, "threads must be a multiple of 64, <= 1024");
+    if (nt % kWave || nt > 512) return fail_with(PRRTC_E_INVALID, "planner threads must be a multiple of 64, <= 512");
 
     PlanArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -753,7 +787,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     size_t lds = 0;
     for (;;) {
         a.ml = model_layout(robot->n_joints, robot->n_robot_spheres, world->n_self_pairs, world->n_world_spheres,
-                            world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt);
+                            world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt / kWave, kFBP);
         lds = planner_lds(a.ml, S, &a.off_expl, &a.off_primes, &a.off_nn, &a.off_hash, &a.off_misc, &a.hash_size);
         if (lds <= static_cast<size_t>(di.max_lds) || nt <= kWave) break;
         nt -= kWave;
