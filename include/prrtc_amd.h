@@ -112,10 +112,13 @@ typedef struct {
     int32_t path_capacity;
 } prrtc_outputs;
 
-/* Launch shape (NULL = defaults chosen by the library). */
+/* Launch shape (NULL = defaults chosen by the library).  Results never depend on it. */
 typedef struct {
-    int32_t threads;  /* threads per work-group, multiple of 64 (planner <= 512, checks <= 1024); 0 = auto */
-    int32_t slots;    /* resident work-groups (queries in flight); 0 = auto */
+    int32_t threads;      /* threads per work-group, multiple of 64 (planner <= 512, checks <= 1024); 0 = auto */
+    int32_t slots;        /* resident work-groups; 0 = all that fit */
+    int32_t pass1_rounds; /* planner: rounds a query runs on one work-group before it is handed to a team of
+                             work-groups; -1 = auto (24 for a batch, 0 for a single query) */
+    int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (32) */
 } prrtc_launch;
 
 /* Workspace bytes a planner call needs (radii + per-query explorer state + queue counters). */

@@ -250,12 +250,68 @@ __device__ __forceinline__ bool lex_less(float da, int ia, float db, int ib) {
     return da < db || (da == db && ia < ib);
 }
 
-// Nearest node (lowest index on ties) of each listed explorer's query among tree[0, size).
+// Nearest node (lowest index on ties) of kEC queries (qsh[c * kMaxDim + d], unused rows zero)
+// among tree[0, size).  Every thread of the work-group calls it; on return red_d[c * 16] and
+// red_i[c * 16] hold query c's squared distance and index.
+template <int MAXD>
+__device__ void nn_tile(const float* __restrict__ tree, int ncap, int size, int dim, const float* qsh, float* red_d,
+                        int* red_i) {
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int nw = (nt + kWave - 1) / kWave;
+    float bd[kEC];
+    int bi[kEC];
+#pragma unroll
+    for (int c = 0; c < kEC; ++c) { bd[c] = FLT_MAX; bi[c] = 0; }
+    for (int i = tid; i < size; i += nt) {
+        float x[MAXD];
+#pragma unroll
+        for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? tree[static_cast<size_t>(d) * ncap + i] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < kEC; ++c) {
+            float dsq = 0.0f;
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) {
+                if (d < dim) {
+                    const float df = x[d] - qsh[c * kMaxDim + d];
+                    dsq = __builtin_fmaf(df, df, dsq);
+                }
+            }
+            if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kEC; ++c) {
+        float d = bd[c];
+        int i = bi[c];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float od = __shfl_xor(d, o, kWave);
+            const int oi = __shfl_xor(i, o, kWave);
+            if (lex_less(od, oi, d, i)) { d = od; i = oi; }
+        }
+        if (lane == 0) { red_d[c * 16 + wid] = d; red_i[c * 16 + wid] = i; }
+    }
+    __syncthreads();
+    if (tid < kEC) {
+        float d = red_d[tid * 16];
+        int i = red_i[tid * 16];
+        for (int w = 1; w < nw; ++w) {
+            const float od = red_d[tid * 16 + w];
+            const int oi = red_i[tid * 16 + w];
+            if (lex_less(od, oi, d, i)) { d = od; i = oi; }
+        }
+        red_d[tid * 16] = d;
+        red_i[tid * 16] = i;
+    }
+    __syncthreads();
+}
+
+// Nearest node of each listed query (query list[k] at qbuf[list[k] * dim]); results to
+// out_i/out_d indexed by the list entry.
 template <int MAXD>
 __device__ void nn_tiles(const float* __restrict__ tree, int ncap, int size, int dim, const int* list, int nl,
                          const float* __restrict__ qbuf, int* out_i, float* out_d, char* nnl) {
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int nw = (nt + kWave - 1) / kWave;
+    const int tid = threadIdx.x, nt = blockDim.x;
     float* qsh = reinterpret_cast<float*>(nnl);
     float* red_d = qsh + kEC * kMaxDim;
     int* red_i = reinterpret_cast<int*>(red_d + kEC * 16);
@@ -266,51 +322,11 @@ __device__ void nn_tiles(const float* __restrict__ tree, int ncap, int size, int
             qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qbuf[list[c0 + c] * dim + d] : 0.0f;
         }
         __syncthreads();
-        float bd[kEC];
-        int bi[kEC];
-#pragma unroll
-        for (int c = 0; c < kEC; ++c) { bd[c] = FLT_MAX; bi[c] = 0; }
-        for (int i = tid; i < size; i += nt) {
-            float x[MAXD];
-#pragma unroll
-            for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? tree[static_cast<size_t>(d) * ncap + i] : 0.0f;
-#pragma unroll
-            for (int c = 0; c < kEC; ++c) {
-                float dsq = 0.0f;
-#pragma unroll
-                for (int d = 0; d < MAXD; ++d) {
-                    if (d < dim) {
-                        const float df = x[d] - qsh[c * kMaxDim + d];
-                        dsq = __builtin_fmaf(df, df, dsq);
-                    }
-                }
-                if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < kEC; ++c) {
-            float d = bd[c];
-            int i = bi[c];
-#pragma unroll
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                const float od = __shfl_xor(d, o, kWave);
-                const int oi = __shfl_xor(i, o, kWave);
-                if (lex_less(od, oi, d, i)) { d = od; i = oi; }
-            }
-            if (lane == 0) { red_d[c * 16 + wid] = d; red_i[c * 16 + wid] = i; }
-        }
-        __syncthreads();
+        nn_tile<MAXD>(tree, ncap, size, dim, qsh, red_d, red_i);
         if (tid < nc) {
-            float d = red_d[tid * 16];
-            int i = red_i[tid * 16];
-            for (int w = 1; w < nw; ++w) {
-                const float od = red_d[tid * 16 + w];
-                const int oi = red_i[tid * 16 + w];
-                if (lex_less(od, oi, d, i)) { d = od; i = oi; }
-            }
             const int e = list[c0 + tid];
-            out_i[e] = i;
-            out_d[e] = d;
+            out_i[e] = red_i[tid * 16];
+            out_d[e] = red_d[tid * 16];
         }
         __syncthreads();
     }

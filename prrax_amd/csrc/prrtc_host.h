@@ -16,6 +16,8 @@ struct DevInfo {
     float wall_khz = 0.0f;
 };
 
+extern thread_local std::string g_err;
+
 // Records msg as the thread's last error and returns code.
 int fail_with(int code, const std::string& msg);
 // Cached device attributes (CU count, LDS per work-group, wall-clock rate).

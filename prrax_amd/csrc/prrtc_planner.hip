@@ -1,9 +1,7 @@
-// prrtc_planner.hip -- persistent lockstep pRRTC planner for MI355X (gfx950) + its C ABI.
+// prrtc_planner.hip -- lockstep pRRTC planner for MI355X (gfx950) + its C ABI.
 //
-// One work-group plans one query at a time and pulls the next query from a device-resident
-// queue (an atomic counter) when it finishes, so a batch of uneven queries runs in a single
-// launch.  Inside a work-group the reference's explorers (its blocks, prrtc_planner.cu:657-674)
-// advance in lockstep rounds:
+// Each query is planned by a team of W work-groups.  The reference's explorers (its blocks,
+// prrtc_planner.cu:657-674) advance in lockstep rounds:
 //
 //   1  sample (Halton, prime shuffle), pick a tree, nearest neighbour, dynamic-domain skip, steer
 //                                                         prrtc_planner.cu:783-881
@@ -18,13 +16,21 @@
 //                                                         prrtc_planner.cu:1020-1062
 //
 // Each round is one legal interleaving of the reference kernel, chosen so that results do not
-// depend on timing: the trees equal the CPU oracle's bit for bit.  Shared state changes only in
-// the commit phases, through prefix sums over explorers (no per-node atomics), which also removes
-// the reference's unordered-append race (a node is counted only after it is written).
+// depend on timing: the trees equal the CPU oracle's bit for bit, whatever the team size.
+// Shared state changes only in the commit phases, through prefix sums over explorers (no
+// per-node atomics), which also removes the reference's unordered-append race.
+//
+// Scheduling (two launches on one stream, no host synchronisation in between):
+//   pass 1  one work-group per query, queries pulled from a device queue; a query still open
+//           after `pass1_rounds` rounds saves its round state and joins the continue list;
+//   pass 2  a cooperative launch: the continue list is split over teams of W work-groups
+//           (W = slots / open queries, a power of two, capped), so the few long queries of an
+//           uneven batch get the whole GPU.  A single query skips pass 1 (budget 0).
+// Inside a team, explorer e belongs to member e % W; work whose result every member needs
+// (prefix sums, sizes, status) is computed redundantly from global state after a team barrier.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -34,7 +40,14 @@
 
 namespace prrtc {
 
-constexpr int kPassMult = 2;   // connect items per pass ~= kPassMult * threads
+constexpr int kPassMult = 2;  // connect items per pass ~= kPassMult * (waves * kFB)
+
+// Per-query explorer arrays kept in global memory ([B][XG_COUNT][S] ints).
+enum { XG_NN, XG_ONN, XG_DSQ, XG_FAIL, XG_VAMP, XG_COUNT };
+// Per-query round state ([B][QS_COUNT] ints) for the pass-1 -> pass-2 hand-over.
+enum { QS_ROUND, QS_SIZE_A, QS_SIZE_B, QS_TIME, QS_COUNT = 8 };
+// Control words ([kCtl] ints).
+enum { CTL_QUEUE, CTL_NCONT, CTL_ERROR, kCtl = 16 };
 
 struct PlanArgs {
     int max_iterations;
@@ -72,19 +85,27 @@ struct PlanArgs {
     float* path;
     int* path_len;
     int path_cap;
-    float* radii;   // [B][2][ncap]
-    float* xbuf;    // [B][3][S][dim]: sample, candidate, connect step
-    int* queue;     // [1]
-    unsigned long long* stats;  // [4]
+    // workspace
+    float* radii;                // [B][2][ncap]
+    int* xg;                     // [B][XG_COUNT][S]
+    int* qstate;                 // [B][QS_COUNT]
+    int* ctl;                    // [kCtl]
+    int* cont;                   // [B]
+    unsigned* tsync;             // [slots][2] (arrivals, generation)
+    unsigned long long* stats;   // [4]
+    // scheduling
+    int pass1_rounds;            // round budget of pass 1 (0: pass 1 only initialises)
+    int wmax;                    // team size cap
+    int slots;
+    // LDS layout
     ModelLayout ml;
-    size_t off_expl, off_primes, off_nn, off_hash, off_misc;
+    size_t off_expl, off_cand, off_primes, off_nn, off_hash, off_misc;
     int hash_size;
     float wall_khz;
 };
 
-// Explorer arrays in LDS (S ints each).
-enum { X_SIDE, X_NN, X_STATE, X_NEWI, X_ONN, X_NEXT, X_FAIL, X_KD, X_VAMP, X_LIST, X_OFFS, X_AUX0, X_AUX1, X_DSQ,
-       X_COUNT };
+// Explorer arrays in each member's LDS (S + 1 ints each).
+enum { X_SIDE, X_NN, X_STATE, X_NEWI, X_ONN, X_NEXT, X_NOK, X_KD, X_VAMP, X_LIST, X_OFFS, X_AUX0, X_AUX1, X_COUNT };
 
 __host__ __device__ inline size_t expl_bytes(int S) { return align16(sizeof(int) * (size_t)(S + 1) * X_COUNT); }
 __host__ __device__ inline int hash_size_for(int S) {
@@ -93,8 +114,10 @@ __host__ __device__ inline int hash_size_for(int S) {
     return h;
 }
 
-enum { SC_QUERY = 0, SC_SIZE_A, SC_SIZE_B, SC_STATUS, SC_EVENT, SC_OVERFLOW, SC_CONN0, SC_CONN1, SC_CONN2, SC_COUNT = 16 };
+enum { SC_QUERY = 0, SC_SIZE_A, SC_SIZE_B, SC_STATUS, SC_EVENT, SC_OVERFLOW, SC_CONN0, SC_CONN1, SC_CONN2, SC_ERR,
+       SC_COUNT = 16 };
 
+// ---------------------------------------------------------------------------------- sampler
 __device__ __forceinline__ float halton(int base, int index) {
     float fraction = 1.0f, result = 0.0f;
     int n = index;
@@ -122,6 +145,13 @@ __device__ __forceinline__ void prime_order(int e, unsigned seed, unsigned char*
     }
 }
 
+// Explorer e's sample of round r in dimension d (prrtc_planner.cu:783-788).
+__device__ __forceinline__ float sample_dim(const float* lo, const float* hi, const unsigned char* pe, int d, int r) {
+    const float val = halton(pe[d & 15], r);
+    return lo[d] + val * (hi[d] - lo[d]);
+}
+
+// ---------------------------------------------------------------------------------- helpers
 template <class F>
 __device__ __forceinline__ int compact(int S, F pred, int* list, int* scan) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -157,13 +187,76 @@ __device__ __forceinline__ void hash_add(int* hk, int* hs, int* hg, int hsize, i
     }
 }
 
+// Global words shared by a team: relaxed agent-scope atomics through address-space-1 pointers.
+__device__ __forceinline__ int gload(const int* p) {
+    return __hip_atomic_load((const PRRTC_GLOBAL int*)(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gstore(int* p, int v) {
+    __hip_atomic_store((PRRTC_GLOBAL int*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gmin(int* p, int v) {
+    __hip_atomic_fetch_min((PRRTC_GLOBAL int*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Team of W work-groups working on one query.
+struct Team {
+    int W, m;
+    unsigned* sync;  // [2]: arrivals, generation
+    int* error;
+    unsigned gen;
+};
+
+// Barrier over the team's work-groups (a plain work-group barrier when W == 1).  Thread 0 of
+// each member releases its work-group's global writes and counts in; the last arrival opens the
+// next generation.  Waiting is bounded (seconds), so a broken team raises the error flag instead
+// of hanging.
+__device__ __forceinline__ void team_barrier(Team& t) {
+    __syncthreads();
+    if (t.W > 1) {
+        if (threadIdx.x == 0) {
+            __threadfence();
+            unsigned* cnt = t.sync;
+            unsigned* gen = t.sync + 1;
+            const unsigned prev = __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)cnt, 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == static_cast<unsigned>(t.W) - 1u) {
+                __hip_atomic_store((PRRTC_GLOBAL unsigned*)cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned spins = 0;
+                while (__hip_atomic_load((PRRTC_GLOBAL unsigned*)gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                       t.gen) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > (1u << 25)) {
+                        gstore(t.error, 1);
+                        break;
+                    }
+                }
+            }
+            __threadfence();
+        }
+        t.gen += 1u;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------- one query
+struct QueryView {
+    int q, N, ng;
+    unsigned seed;
+    float *TA, *TB, *RA, *RB;
+    int *PA, *PB;
+    int *g_nn, *g_onn, *g_fail, *g_vamp;
+    float* g_dsq;
+    int* qs;
+};
+
 template <int MAXD>
-__device__ void plan_query(const PlanArgs& a, int q, char* lds) {
+__device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team& team, int r_begin, int r_budget,
+                          unsigned long long t0) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int dim = a.dim, S = a.S, G = a.G, ncap = a.ncap;
-    const int ng = a.goal_counts ? max(1, min(a.goal_counts[q], a.gmax)) : a.gmax;
-    const int N = a.max_nodes_q ? min(max(a.max_nodes_q[q], ng), ncap) : ncap;
-    const unsigned seed = a.seeds ? a.seeds[q] : 0u;
+    const int lane = tid & (kWave - 1), wid = tid / kWave, nwaves = nt / kWave;
+    const int dim = a.dim, S = a.S, G = a.G, ncap = a.ncap, N = V.N, W = team.W, m = team.m;
     int* X = reinterpret_cast<int*>(lds + a.off_expl);
     int* side = X + X_SIDE * (S + 1);
     int* nn = X + X_NN * (S + 1);
@@ -171,74 +264,68 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
     int* newi = X + X_NEWI * (S + 1);
     int* onn = X + X_ONN * (S + 1);
     int* next = X + X_NEXT * (S + 1);
-    int* fail = X + X_FAIL * (S + 1);
+    int* nok = X + X_NOK * (S + 1);
     int* kd = X + X_KD * (S + 1);
     int* vamp = X + X_VAMP * (S + 1);
     int* list = X + X_LIST * (S + 1);
     int* offs = X + X_OFFS * (S + 1);
     int* aux0 = X + X_AUX0 * (S + 1);
     int* aux1 = X + X_AUX1 * (S + 1);
-    float* dsq = reinterpret_cast<float*>(X + X_DSQ * (S + 1));
+    float* cand = reinterpret_cast<float*>(lds + a.off_cand);
+    float* vec = cand + S * dim;
     unsigned char* primes = reinterpret_cast<unsigned char*>(lds + a.off_primes);
     char* nnl = lds + a.off_nn;
+    float* qsh = reinterpret_cast<float*>(nnl);
+    float* red_d = qsh + kEC * kMaxDim;
+    int* red_i = reinterpret_cast<int*>(red_d + kEC * 16);
     int* hk = reinterpret_cast<int*>(lds + a.off_hash);
     int* hs = hk + a.hash_size;
     int* hg = hs + a.hash_size;
     int* scan = reinterpret_cast<int*>(lds + a.off_misc);
     int* sc = scan + 20;
 
-    const unsigned long long t0 = wall_clock64();
     Model M;
-    stage_model(M, lds, a.ml, a.robot, a.world, q, a.dim, a.mode, a.margin, scan);
-    const int lane = tid & (kWave - 1), wid = tid / kWave, nwaves = nt / kWave;
-    const WaveArea W = wave_area(lds, a.ml, M.n_joints);
+    stage_model(M, lds, a.ml, a.robot, a.world, V.q, a.dim, a.mode, a.margin, scan);
+    const WaveArea WA = wave_area(lds, a.ml, M.n_joints);
     LaneScratch L;  // FK slot of this lane (lanes < kFB) in the wave's FK area
-    L.q = W.q + lane;
-    L.T = W.T + lane;
+    L.q = WA.q + lane;
+    L.T = WA.T + lane;
     L.nt = kFBP;
+    float* TA = V.TA;
+    float* TB = V.TB;
+    float* RA = V.RA;
+    float* RB = V.RB;
+    int* PA = V.PA;
+    int* PB = V.PB;
 
-    float* TA = a.tree_a + static_cast<size_t>(q) * dim * ncap;
-    float* TB = a.tree_b + static_cast<size_t>(q) * dim * ncap;
-    int* PA = a.par_a + static_cast<size_t>(q) * ncap;
-    int* PB = a.par_b + static_cast<size_t>(q) * ncap;
-    float* RA = a.radii + static_cast<size_t>(q) * 2 * ncap;
-    float* RB = RA + ncap;
-    float* samp = a.xbuf + static_cast<size_t>(q) * 3 * S * dim;
-    float* cand = samp + S * dim;
-    float* vec = cand + S * dim;
-    const float* start = a.starts + static_cast<size_t>(q) * dim;
-    const float* goals = a.goals + static_cast<size_t>(q) * a.gmax * dim;
-
-    for (int d = tid; d < dim; d += nt) TA[static_cast<size_t>(d) * ncap] = start[d];
-    for (int i = tid; i < ng * dim; i += nt) {
-        const int g = i / dim, d = i - (i / dim) * dim;
-        TB[static_cast<size_t>(d) * ncap + g] = goals[i];
+    for (int e = tid; e < S; e += nt) {
+        prime_order(e, V.seed, primes + 16 * e);
+        vamp[e] = V.g_vamp[e];
     }
-    if (tid == 0) { PA[0] = 0; RA[0] = FLT_MAX; }
-    for (int g = tid; g < ng; g += nt) { PB[g] = g; RB[g] = FLT_MAX; }
-    for (int e = tid; e < S; e += nt) { prime_order(e, seed, primes + 16 * e); vamp[e] = 0; }
     for (int h = tid; h < a.hash_size; h += nt) { hk[h] = -1; hs[h] = 0; hg[h] = 0; }
     if (tid == 0) {
-        sc[SC_SIZE_A] = 1; sc[SC_SIZE_B] = ng; sc[SC_STATUS] = 0;
+        sc[SC_SIZE_A] = V.qs[QS_SIZE_A];
+        sc[SC_SIZE_B] = V.qs[QS_SIZE_B];
+        sc[SC_STATUS] = 0;
         sc[SC_CONN0] = -1; sc[SC_CONN1] = -1; sc[SC_CONN2] = -1;
+        sc[SC_ERR] = 0;
     }
     __syncthreads();
 
     unsigned long long evals = 0, flops = 0;  // lane 0 of each wave counts
-    int r = 0;
-    for (r = 1;; ++r) {
+    const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
+    int r = r_begin;
+    for (;; ++r) {
         if (r > a.max_iterations) {
             if (tid == 0) sc[SC_STATUS] = PRRTC_STATUS_ITER_CAP;
+            __syncthreads();
             break;
         }
+        if (r >= r_end) break;  // pass-1 budget spent: hand over to a team
         const int size_a = sc[SC_SIZE_A], size_b = sc[SC_SIZE_B];
-        // ---- phase 1a: samples and tree side
+        // ---- phase 1a: tree side of every explorer (each member keeps a full copy); owners clear
+        // the shared collision flags of their explorers for this round.
         for (int e = tid; e < S; e += nt) {
-            const unsigned char* pe = primes + 16 * e;
-            for (int d = 0; d < dim; ++d) {
-                const float val = halton(pe[d & 15], r);
-                samp[e * dim + d] = a.min_vals[d] + val * (a.max_vals[d] - a.min_vals[d]);
-            }
             int t;
             const int total = size_a + size_b;
             if (a.balance_mode == 0 || r == 1) {
@@ -262,47 +349,66 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                 t = (size_a <= size_b) ? 0 : 1;
             }
             side[e] = t;
+            if (e % W == m) gstore(&V.g_fail[e], 0);
         }
         __syncthreads();
-        // ---- phase 1b: nearest neighbour in the chosen tree
-        {
-            const int n0 = compact(S, [&](int e) { return side[e] == 0; }, list, scan);
-            nn_tiles<MAXD>(TA, ncap, size_a, dim, list, n0, samp, nn, dsq, nnl);
-            const int n1 = compact(S, [&](int e) { return side[e] == 1; }, list, scan);
-            nn_tiles<MAXD>(TB, ncap, size_b, dim, list, n1, samp, nn, dsq, nnl);
+        // ---- phase 1b: nearest neighbour of each explorer's sample in its tree; tiles of kEC
+        // explorers, tile c on member c % W
+        for (int t = 0; t < 2; ++t) {
+            const int nl = compact(S, [&](int e) { return side[e] == t; }, list, scan);
+            for (int c0 = m * kEC; c0 < nl; c0 += W * kEC) {
+                const int nc = min(kEC, nl - c0);
+                for (int k = tid; k < kEC * MAXD; k += nt) {
+                    const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
+                    qsh[c * kMaxDim + d] = (c < nc && d < dim)
+                        ? sample_dim(a.min_vals, a.max_vals, primes + 16 * list[c0 + c], d, r) : 0.0f;
+                }
+                __syncthreads();
+                nn_tile<MAXD>(t ? TB : TA, ncap, t ? size_b : size_a, dim, qsh, red_d, red_i);
+                if (tid < nc) {
+                    const int e = list[c0 + tid];
+                    gstore(&V.g_nn[e], red_i[tid * 16]);
+                    gstore(reinterpret_cast<int*>(&V.g_dsq[e]), __float_as_int(red_d[tid * 16]));
+                }
+                __syncthreads();
+            }
         }
-        // ---- phase 1c: dynamic-domain skip and steer
+        team_barrier(team);  // B1: nearest neighbours of all explorers are in global memory
+        // ---- phase 1c: dynamic-domain skip and steer, every explorer (redundant per member)
         for (int e = tid; e < S; e += nt) {
             const int t = side[e];
             const float* T = t ? TB : TA;
             const float* R = t ? RB : RA;
-            const float nd = __builtin_sqrtf(dsq[e]);
-            if (a.dd && R[nn[e]] < nd) {
+            const int ni = gload(&V.g_nn[e]);
+            nn[e] = ni;
+            const float nd = __builtin_sqrtf(__int_as_float(gload(reinterpret_cast<const int*>(&V.g_dsq[e]))));
+            if (a.dd && R[ni] < nd) {
                 state[e] = 0;
             } else {
                 const float s = (nd > 1e-8f) ? fminf(1.0f, a.step_size / nd) : 1.0f;
                 for (int d = 0; d < dim; ++d) {
-                    const float base = T[static_cast<size_t>(d) * ncap + nn[e]];
-                    cand[e * dim + d] = base + (samp[e * dim + d] - base) * s;
+                    const float smp = sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r);
+                    const float base = T[static_cast<size_t>(d) * ncap + ni];
+                    cand[e * dim + d] = base + (smp - base) * s;
                 }
                 state[e] = 1;
             }
-            fail[e] = 0;
         }
         __syncthreads();
         // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Each wave takes kFB
-        // (explorer, waypoint) items, computes their FK one per lane, then checks them one by one.
+        // (explorer, waypoint) items, computes their FK one per lane, then checks them one by one;
+        // the team's waves share the item batches.
         {
             const int nv = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
             const int items = nv * G;
             const int nb = (items + kFB - 1) / kFB;
-            for (int b = wid; b < nb; b += nwaves) {
+            for (int b = m * nwaves + wid; b < nb; b += W * nwaves) {
                 const int it = b * kFB + lane;
                 int e = -1;
                 if (lane < kFB && it < items) {
                     const int slot = it / G, k = it - slot * G;
                     e = list[slot];
-                    if (lds_load(&fail[e])) {
+                    if (gload(&V.g_fail[e])) {
                         e = -1;
                     } else {
                         const float* T = side[e] ? TB : TA;
@@ -317,18 +423,18 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                 wave_sync();
                 for (int c = 0; c < kFB; ++c) {
                     const int ec = uniform(__shfl(e, c, kWave));
-                    if (ec < 0 || uniform(lds_load(&fail[ec]))) continue;
+                    if (ec < 0 || uniform(gload(&V.g_fail[ec]))) continue;
                     if (lane == 0) ++evals;
-                    if (wave_in_collision(M, W.T + c, kFBP, flops) && lane == 0) lds_store(&fail[ec], 1);
+                    if (wave_in_collision(M, WA.T + c, kFBP, flops) && lane == 0) gstore(&V.g_fail[ec], 1);
                 }
                 wave_sync();
             }
-            __syncthreads();
         }
-        // ---- phase 3: commit -- appends in explorer order, then radius updates
+        team_barrier(team);  // B3: every extend verdict is in global memory
+        // ---- phase 3: commit -- appends in explorer order (prefix sums), radius updates
         {
             for (int e = tid; e < S; e += nt)
-                if (state[e] == 1 && fail[e]) state[e] = 3;  // 3: collided
+                if (state[e] == 1 && gload(&V.g_fail[e])) state[e] = 3;  // 3: collided
             if (tid == 0) sc[SC_OVERFLOW] = 0;
             __syncthreads();
             int base0 = size_a, base1 = size_b;
@@ -344,11 +450,13 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                     if (fr) {
                         const int idx = t ? base1 + p1 : base0 + p0;
                         if (idx < N) {
-                            float* T = t ? TB : TA;
-                            for (int d = 0; d < dim; ++d) T[static_cast<size_t>(d) * ncap + idx] = cand[e * dim + d];
-                            (t ? PB : PA)[idx] = nn[e];
-                            (t ? RB : RA)[idx] = FLT_MAX;
                             ni = idx;
+                            if (e % W == m) {
+                                float* T = t ? TB : TA;
+                                for (int d = 0; d < dim; ++d) T[static_cast<size_t>(d) * ncap + idx] = cand[e * dim + d];
+                                (t ? PB : PA)[idx] = nn[e];
+                                (t ? RB : RA)[idx] = FLT_MAX;
+                            }
                         } else {
                             sc[SC_OVERFLOW] = 1;
                         }
@@ -358,7 +466,7 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                 base0 += tot0;
                 base1 += tot1;
             }
-            if (a.dd) {
+            if (a.dd && m == 0) {
                 for (int e = tid; e < S; e += nt) {
                     if (state[e] == 3) hash_add(hk, hs, hg, a.hash_size, nn[e] * 2 + side[e], true);
                     else if (newi[e] >= 0) hash_add(hk, hs, hg, a.hash_size, nn[e] * 2 + side[e], false);
@@ -381,36 +489,55 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                 sc[SC_SIZE_B] = min(base1, N);
                 if (sc[SC_OVERFLOW]) sc[SC_STATUS] = PRRTC_STATUS_TREE_FULL;
             }
-            __syncthreads();
-            if (sc[SC_STATUS] != 0) break;
         }
+        team_barrier(team);  // B4: new nodes and radii are in global memory
+        if (sc[SC_STATUS] != 0) break;
         // ---- phase 4: connect nearest neighbour in the other tree
         {
             const int sa = sc[SC_SIZE_A], sb = sc[SC_SIZE_B];
-            const int n0 = compact(S, [&](int e) { return newi[e] >= 0 && side[e] == 1; }, list, scan);
-            nn_tiles<MAXD>(TA, ncap, sa, dim, list, n0, cand, onn, dsq, nnl);
-            const int n1 = compact(S, [&](int e) { return newi[e] >= 0 && side[e] == 0; }, list, scan);
-            nn_tiles<MAXD>(TB, ncap, sb, dim, list, n1, cand, onn, dsq, nnl);
-            for (int e = tid; e < S; e += nt) {
-                if (newi[e] < 0) continue;
-                const float* O = side[e] ? TA : TB;
-                const float cd = __builtin_sqrtf(dsq[e]);
-                int n_ext = 0;
-                if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
-                next[e] = n_ext;
-                fail[e] = n_ext;
-                kd[e] = 0;
-                for (int d = 0; d < dim; ++d)
-                    vec[e * dim + d] = (n_ext > 0)
-                        ? (O[static_cast<size_t>(d) * ncap + onn[e]] - cand[e * dim + d]) / static_cast<float>(n_ext)
-                        : 0.0f;
+            for (int t = 0; t < 2; ++t) {  // t: tree searched (the other tree of the extended one)
+                const int nl = compact(S, [&](int e) { return newi[e] >= 0 && side[e] == 1 - t; }, list, scan);
+                for (int c0 = m * kEC; c0 < nl; c0 += W * kEC) {
+                    const int nc = min(kEC, nl - c0);
+                    for (int k = tid; k < kEC * MAXD; k += nt) {
+                        const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
+                        qsh[c * kMaxDim + d] = (c < nc && d < dim) ? cand[list[c0 + c] * dim + d] : 0.0f;
+                    }
+                    __syncthreads();
+                    nn_tile<MAXD>(t ? TB : TA, ncap, t ? sb : sa, dim, qsh, red_d, red_i);
+                    if (tid < nc) {
+                        const int e = list[c0 + tid];
+                        gstore(&V.g_onn[e], red_i[tid * 16]);
+                        gstore(reinterpret_cast<int*>(&V.g_dsq[e]), __float_as_int(red_d[tid * 16]));
+                    }
+                    __syncthreads();
+                }
             }
-            __syncthreads();
         }
-        // ---- phase 5: connect sub-edges in passes; fail[e] = first failing sub-edge (or n_ext)
+        team_barrier(team);  // B5: connect nearest neighbours are in global memory
+        for (int e = tid; e < S; e += nt) {
+            if (newi[e] < 0) continue;
+            const float* O = side[e] ? TA : TB;
+            const int oi = gload(&V.g_onn[e]);
+            onn[e] = oi;
+            const float cd = __builtin_sqrtf(__int_as_float(gload(reinterpret_cast<const int*>(&V.g_dsq[e]))));
+            int n_ext = 0;
+            if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
+            next[e] = n_ext;
+            kd[e] = 0;
+            for (int d = 0; d < dim; ++d)
+                vec[e * dim + d] = (n_ext > 0)
+                    ? (O[static_cast<size_t>(d) * ncap + oi] - cand[e * dim + d]) / static_cast<float>(n_ext)
+                    : 0.0f;
+            if (e % W == m) gstore(&V.g_fail[e], n_ext);
+        }
+        __syncthreads();
+        // ---- phase 5: each member walks the connect sub-edges of its own explorers in passes;
+        // g_fail[e] ends as the first failing sub-edge (or n_ext).
         for (;;) {
-            const int na = compact(S, [&](int e) { return newi[e] >= 0 && fail[e] == next[e] && kd[e] < next[e]; },
-                                   list, scan);
+            const int na = compact(S, [&](int e) {
+                return e % W == m && newi[e] >= 0 && kd[e] < next[e] && gload(&V.g_fail[e]) == next[e];
+            }, list, scan);
             if (na == 0) break;
             const int mper = max(1, (nwaves * kFB * kPassMult) / (na * G));
             int run = 0;
@@ -442,7 +569,7 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                     const int local = it - offs[lo];
                     j = kd[e] + local / G;
                     const int k = local - (local / G) * G;
-                    if (j >= lds_load(&fail[e])) {
+                    if (j >= gload(&V.g_fail[e])) {
                         e = -1;
                     } else {
                         const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
@@ -459,9 +586,9 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                 for (int c = 0; c < kFB; ++c) {
                     const int ec = uniform(__shfl(e, c, kWave));
                     const int jc = uniform(__shfl(j, c, kWave));
-                    if (ec < 0 || jc >= uniform(lds_load(&fail[ec]))) continue;
+                    if (ec < 0 || jc >= uniform(gload(&V.g_fail[ec]))) continue;
                     if (lane == 0) ++evals;
-                    if (wave_in_collision(M, W.T + c, kFBP, flops) && lane == 0) lds_min(&fail[ec], jc);
+                    if (wave_in_collision(M, WA.T + c, kFBP, flops) && lane == 0) gmin(&V.g_fail[ec], jc);
                 }
                 wave_sync();
             }
@@ -472,8 +599,10 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
             }
             __syncthreads();
         }
+        team_barrier(team);  // B7: every connect walk's length is in global memory
         // ---- phase 6: connect appends (explorer-major); first success or overflow decides
         {
+            for (int e = tid; e < S; e += nt) nok[e] = (newi[e] >= 0) ? gload(&V.g_fail[e]) : 0;
             if (tid == 0) sc[SC_EVENT] = S;
             __syncthreads();
             int base0 = sc[SC_SIZE_A], base1 = sc[SC_SIZE_B];
@@ -481,33 +610,34 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                 const int e = e0 + tid;
                 const bool act = e < S && newi[e] >= 0;
                 const int t = act ? side[e] : 0;
-                const int nok = act ? fail[e] : 0;
+                const int n_ok = act ? nok[e] : 0;
                 int tot0, tot1;
-                const int p0 = block_excl_scan(t == 0 ? nok : 0, scan, &tot0);
-                const int p1 = block_excl_scan(t == 1 ? nok : 0, scan, &tot1);
+                const int p0 = block_excl_scan(t == 0 ? n_ok : 0, scan, &tot0);
+                const int p1 = block_excl_scan(t == 1 ? n_ok : 0, scan, &tot1);
                 if (act) {
                     const int b = t ? base1 + p1 : base0 + p0;
-                    float* T = t ? TB : TA;
-                    int* P = t ? PB : PA;
-                    float* R = t ? RB : RA;
-                    int parent = newi[e];
-                    bool over = false;
-                    for (int k = 0; k < nok; ++k) {
-                        const int idx = b + k;
-                        if (idx >= N) { over = true; break; }
-                        for (int d = 0; d < dim; ++d) {
-                            float c = cand[e * dim + d];
-                            const float v = vec[e * dim + d];
-                            for (int jj = 0; jj <= k; ++jj) c += v;
-                            T[static_cast<size_t>(d) * ncap + idx] = c;
+                    const bool over = b + n_ok > N;
+                    if (e % W == m) {
+                        float* T = t ? TB : TA;
+                        int* P = t ? PB : PA;
+                        float* R = t ? RB : RA;
+                        int parent = newi[e];
+                        for (int k = 0; k < n_ok && b + k < N; ++k) {
+                            const int idx = b + k;
+                            for (int d = 0; d < dim; ++d) {
+                                float c = cand[e * dim + d];
+                                const float v = vec[e * dim + d];
+                                for (int jj = 0; jj <= k; ++jj) c += v;
+                                T[static_cast<size_t>(d) * ncap + idx] = c;
+                            }
+                            P[idx] = parent;
+                            R[idx] = FLT_MAX;
+                            parent = idx;
                         }
-                        P[idx] = parent;
-                        R[idx] = FLT_MAX;
-                        parent = idx;
                     }
                     aux0[e] = over ? 1 : 0;
-                    aux1[e] = parent;
-                    if (over || nok == next[e]) lds_min(&sc[SC_EVENT], e);
+                    aux1[e] = (n_ok > 0) ? b + n_ok - 1 : newi[e];
+                    if (over || n_ok == next[e]) lds_min(&sc[SC_EVENT], e);
                 }
                 base0 += tot0;
                 base1 += tot1;
@@ -528,18 +658,40 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
                     }
                 }
             }
-            __syncthreads();
-            if (sc[SC_STATUS] != 0) break;
         }
+        team_barrier(team);  // B8: connect nodes are in global memory
+        if (W > 1) {
+            if (tid == 0) sc[SC_ERR] = gload(team.error);
+            __syncthreads();
+        }
+        if (sc[SC_STATUS] != 0 || sc[SC_ERR]) break;
     }
-    // ---- outputs
+    // ---- counters
     if (lane == 0 && evals) {
         global_add(&a.stats[0], evals);
         global_add(&a.stats[2], flops);
     }
-    const int st = sc[SC_STATUS];
-    if (tid == 0) {
-        const unsigned long long t1 = wall_clock64();
+    const int st = sc[SC_ERR] ? -1 : sc[SC_STATUS];
+    const unsigned long long t1 = wall_clock64();
+    const float dt = static_cast<float>(static_cast<double>(t1 - t0) / a.wall_khz);
+    if (st == 0) {
+        // pass-1 budget spent: save the round state (every member holds the same values)
+        if (m == 0) {
+            for (int e = tid; e < S; e += nt) V.g_vamp[e] = vamp[e];
+            if (tid == 0) {
+                V.qs[QS_ROUND] = r;
+                V.qs[QS_SIZE_A] = sc[SC_SIZE_A];
+                V.qs[QS_SIZE_B] = sc[SC_SIZE_B];
+                V.qs[QS_TIME] = __float_as_int(__int_as_float(V.qs[QS_TIME]) + dt);
+                const int k = global_add(&a.ctl[CTL_NCONT], 1);
+                a.cont[k] = V.q;
+            }
+        }
+        __syncthreads();
+        return;
+    }
+    if (m == 0 && tid == 0) {
+        const int q = V.q;
         a.sizes[2 * q] = sc[SC_SIZE_A];
         a.sizes[2 * q + 1] = sc[SC_SIZE_B];
         a.completed[2 * q] = sc[SC_SIZE_A];
@@ -549,7 +701,7 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
         a.conn[3 * q + 1] = sc[SC_CONN1];
         a.conn[3 * q + 2] = sc[SC_CONN2];
         a.solved[q] = (st == PRRTC_STATUS_SOLVED) ? 1 : -1;
-        a.time_ms[q] = static_cast<float>(static_cast<double>(t1 - t0) / a.wall_khz);
+        a.time_ms[q] = __int_as_float(V.qs[QS_TIME]) + dt;
         if (a.status) a.status[q] = st;
         global_add(&a.stats[1], static_cast<unsigned long long>(r));
         if (a.path) {
@@ -593,136 +745,160 @@ __device__ void plan_query(const PlanArgs& a, int q, char* lds) {
     __syncthreads();
 }
 
+__device__ QueryView query_view(const PlanArgs& a, int q) {
+    QueryView V;
+    V.q = q;
+    V.ng = a.goal_counts ? max(1, min(a.goal_counts[q], a.gmax)) : a.gmax;
+    V.N = a.max_nodes_q ? min(max(a.max_nodes_q[q], V.ng), a.ncap) : a.ncap;
+    V.seed = a.seeds ? a.seeds[q] : 0u;
+    V.TA = a.tree_a + static_cast<size_t>(q) * a.dim * a.ncap;
+    V.TB = a.tree_b + static_cast<size_t>(q) * a.dim * a.ncap;
+    V.PA = a.par_a + static_cast<size_t>(q) * a.ncap;
+    V.PB = a.par_b + static_cast<size_t>(q) * a.ncap;
+    V.RA = a.radii + static_cast<size_t>(q) * 2 * a.ncap;
+    V.RB = V.RA + a.ncap;
+    int* xg = a.xg + static_cast<size_t>(q) * XG_COUNT * a.S;
+    V.g_nn = xg + XG_NN * a.S;
+    V.g_onn = xg + XG_ONN * a.S;
+    V.g_dsq = reinterpret_cast<float*>(xg + XG_DSQ * a.S);
+    V.g_fail = xg + XG_FAIL * a.S;
+    V.g_vamp = xg + XG_VAMP * a.S;
+    V.qs = a.qstate + static_cast<size_t>(q) * QS_COUNT;
+    return V;
+}
+
+// Roots and round state of a new query (prrtc_planner.cu:97-125, 1179-1252).
+__device__ void init_query(const PlanArgs& a, const QueryView& V) {
+    const int tid = threadIdx.x, nt = blockDim.x, dim = a.dim, ncap = a.ncap;
+    const float* start = a.starts + static_cast<size_t>(V.q) * dim;
+    const float* goals = a.goals + static_cast<size_t>(V.q) * a.gmax * dim;
+    for (int d = tid; d < dim; d += nt) V.TA[static_cast<size_t>(d) * ncap] = start[d];
+    for (int i = tid; i < V.ng * dim; i += nt) {
+        const int g = i / dim, d = i - (i / dim) * dim;
+        V.TB[static_cast<size_t>(d) * ncap + g] = goals[i];
+    }
+    if (tid == 0) {
+        V.PA[0] = 0;
+        V.RA[0] = FLT_MAX;
+        V.qs[QS_ROUND] = 1;
+        V.qs[QS_SIZE_A] = 1;
+        V.qs[QS_SIZE_B] = V.ng;
+        V.qs[QS_TIME] = __float_as_int(0.0f);
+    }
+    for (int g = tid; g < V.ng; g += nt) { V.PB[g] = g; V.RB[g] = FLT_MAX; }
+    for (int e = tid; e < a.S; e += nt) V.g_vamp[e] = 0;
+    __syncthreads();
+}
+
+// Pass 1: one work-group per query from the device queue.
 template <int MAXD>
-__global__ __launch_bounds__(512) void plan_kernel(PlanArgs a) {
+__global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
     int* sc = reinterpret_cast<int*>(lds + a.off_misc) + 20;
+    Team team{1, 0, nullptr, a.ctl + CTL_ERROR, 0u};
     for (;;) {
-        if (threadIdx.x == 0) sc[SC_QUERY] = global_add(a.queue, 1);
+        if (threadIdx.x == 0) sc[SC_QUERY] = global_add(a.ctl + CTL_QUEUE, 1);
         __syncthreads();
         const int qi = sc[SC_QUERY];
         __syncthreads();
         if (qi >= a.B) return;
-        const int q = a.order ? a.order[qi] : qi;
-        plan_query<MAXD>(a, q, lds);
+        const unsigned long long t0 = wall_clock64();
+        const QueryView V = query_view(a, a.order ? a.order[qi] : qi);
+        init_query(a, V);
+        run_query<MAXD>(a, V, lds, team, 1, a.pass1_rounds, t0);
     }
 }
 
-__global__ void reset_kernel(int* queue, unsigned long long* stats) {
-    if (threadIdx.x == 0) *queue = 0;
-    if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
+// Pass 2: teams of W work-groups resume the queries left open by pass 1.
+template <int MAXD>
+__global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    const int ncont = a.ctl[CTL_NCONT];
+    if (ncont == 0) return;
+    int W = 1;
+    while (W * 2 <= a.wmax && W * 2 * ncont <= a.slots) W *= 2;
+    const int b = blockIdx.x;
+    int team_id, member;
+    if (a.slots % (8 * W) == 0) {  // keep a team on one XCD (work-groups are dealt round-robin)
+        team_id = (b % 8) + 8 * ((b / 8) / W);
+        member = (b / 8) % W;
+    } else {
+        team_id = b / W;
+        member = b % W;
+    }
+    const int nteams = a.slots / W;
+    if (team_id >= nteams) return;
+    Team team{W, member, a.tsync + 2 * team_id, a.ctl + CTL_ERROR, 0u};
+    for (int k = team_id; k < ncont; k += nteams) {
+        const unsigned long long t0 = wall_clock64();
+        const QueryView V = query_view(a, a.cont[k]);
+        run_query<MAXD>(a, V, lds, team, V.qs[QS_ROUND], -1, t0);
+        team_barrier(team);  // members leave the query together before the next one
+    }
+}
+
+__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, int nsync) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < kCtl) ctl[t] = 0;
+    if (t < 4) stats[t] = 0ull;
+    for (int i = t; i < 2 * nsync; i += gridDim.x * blockDim.x) tsync[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------------- host side
-thread_local std::string g_err;
-
-int fail_with(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-DevInfo& dev_info(int dev) {
-    static DevInfo info[64];
-    DevInfo& d = info[dev & 63];
-    if (!d.ok) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) v = 1;
-        d.cus = v;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-        d.max_lds = v;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, dev) != hipSuccess) v = 100000;
-        d.wall_khz = static_cast<float>(v > 0 ? v : 100000);
-        d.ok = 1;
-    }
-    return d;
-}
-
 struct WsLayout {
-    size_t radii, xbuf, queue, stats, total;
+    size_t radii, xg, qstate, ctl, cont, tsync, stats, total;
 };
 
-WsLayout ws_layout(int B, int dim, int ncap, int S) {
+constexpr int kMaxSlots = 8192;
+
+WsLayout ws_layout(int B, int ncap, int S) {
     WsLayout w;
     size_t o = 0;
     w.radii = o; o = align16(o + sizeof(float) * 2 * static_cast<size_t>(ncap) * B);
-    w.xbuf = o; o = align16(o + sizeof(float) * 3 * static_cast<size_t>(S) * dim * B);
-    w.queue = o; o = align16(o + 16);
+    w.xg = o; o = align16(o + sizeof(int) * XG_COUNT * static_cast<size_t>(S) * B);
+    w.qstate = o; o = align16(o + sizeof(int) * QS_COUNT * static_cast<size_t>(B));
+    w.ctl = o; o = align16(o + sizeof(int) * kCtl);
+    w.cont = o; o = align16(o + sizeof(int) * static_cast<size_t>(B));
+    w.tsync = o; o = align16(o + sizeof(unsigned) * 2 * kMaxSlots);
     w.stats = o; o = align16(o + 4 * sizeof(unsigned long long));
     w.total = o + 256;
     return w;
 }
 
-size_t planner_lds(const ModelLayout& ml, int S, size_t* off_expl, size_t* off_primes, size_t* off_nn,
-                   size_t* off_hash, size_t* off_misc, int* hsize) {
+size_t planner_lds(const ModelLayout& ml, int S, int dim, PlanArgs& a) {
     size_t o = ml.end;
-    *off_expl = o; o = align16(o + expl_bytes(S));
-    *off_primes = o; o = align16(o + 16 * static_cast<size_t>(S));
-    *off_nn = o; o = align16(o + nn_bytes());
-    *hsize = hash_size_for(S);
-    *off_hash = o; o = align16(o + 3 * sizeof(int) * static_cast<size_t>(*hsize));
-    *off_misc = o; o = align16(o + sizeof(int) * (20 + SC_COUNT));
+    a.off_expl = o; o = align16(o + expl_bytes(S));
+    a.off_cand = o; o = align16(o + sizeof(float) * 2 * static_cast<size_t>(S) * dim);
+    a.off_primes = o; o = align16(o + 16 * static_cast<size_t>(S));
+    a.off_nn = o; o = align16(o + nn_bytes());
+    a.hash_size = hash_size_for(S);
+    a.off_hash = o; o = align16(o + 3 * sizeof(int) * static_cast<size_t>(a.hash_size));
+    a.off_misc = o; o = align16(o + sizeof(int) * (20 + SC_COUNT));
     return o;
-}
-
-RobotDev robot_dev(const prrtc_robot& r) {
-    RobotDev d;
-    d.twists = r.fk_twists; d.parent_tf = r.fk_parent_tf; d.parent_idx = r.fk_parent_idx; d.act_idx = r.fk_act_idx;
-    d.mimic_mul = r.fk_mimic_mul; d.mimic_off = r.fk_mimic_off; d.mimic_act_idx = r.fk_mimic_act_idx;
-    d.topo_inv = r.fk_topo_inv; d.n_joints = r.n_joints; d.sphere_link_idx = r.sphere_link_idx;
-    d.sphere_local = r.sphere_local; d.sphere_radius = r.sphere_radius; d.n_spheres = r.n_robot_spheres;
-    return d;
-}
-
-WorldDev world_dev(const prrtc_world& w, bool per_query) {
-    WorldDev d;
-    d.ws = w.world_spheres; d.wc = w.world_capsules; d.wb = w.world_boxes; d.wh = w.world_halfspaces; d.sp = w.self_pairs;
-    d.n_ws = w.n_world_spheres; d.n_wc = w.n_world_capsules; d.n_wb = w.n_world_boxes; d.n_wh = w.n_world_halfspaces;
-    d.n_sp = w.n_self_pairs;
-    d.s_ws = per_query ? 4LL * d.n_ws : 0; d.s_wc = per_query ? 7LL * d.n_wc : 0; d.s_wb = per_query ? 15LL * d.n_wb : 0;
-    d.s_wh = per_query ? 6LL * d.n_wh : 0; d.s_sp = per_query ? 2LL * d.n_sp : 0;
-    d.counts = per_query ? w.counts : nullptr;
-    return d;
-}
-
-int check_robot(const prrtc_robot* r) {
-    if (!r) return fail_with(PRRTC_E_INVALID, "robot is null");
-    if (r->n_joints < 0 || r->n_joints > PRRTC_MAX_JOINTS)
-        return fail_with(PRRTC_E_INVALID, "n_joints must be in [0, 64]");
-    if (r->n_robot_spheres < 0) return fail_with(PRRTC_E_INVALID, "n_robot_spheres < 0");
-    if (r->n_joints > 0 && (!r->fk_twists || !r->fk_parent_tf || !r->fk_parent_idx || !r->fk_act_idx ||
-                            !r->fk_mimic_mul || !r->fk_mimic_off || !r->fk_mimic_act_idx || !r->fk_topo_inv))
-        return fail_with(PRRTC_E_INVALID, "FK arrays must be non-null when n_joints > 0");
-    if (r->n_robot_spheres > 0 && (!r->sphere_link_idx || !r->sphere_local || !r->sphere_radius))
-        return fail_with(PRRTC_E_INVALID, "sphere arrays must be non-null when n_robot_spheres > 0");
-    return PRRTC_OK;
-}
-
-int check_world(const prrtc_world* w) {
-    if (!w) return fail_with(PRRTC_E_INVALID, "world is null");
-    if (w->n_world_spheres < 0 || w->n_world_capsules < 0 || w->n_world_boxes < 0 || w->n_world_halfspaces < 0 ||
-        w->n_self_pairs < 0)
-        return fail_with(PRRTC_E_INVALID, "negative world count");
-    if ((w->n_world_spheres && !w->world_spheres) || (w->n_world_capsules && !w->world_capsules) ||
-        (w->n_world_boxes && !w->world_boxes) || (w->n_world_halfspaces && !w->world_halfspaces) ||
-        (w->n_self_pairs && !w->self_pairs))
-        return fail_with(PRRTC_E_INVALID, "world array null with nonzero count");
-    return PRRTC_OK;
-}
-
-template <int MAXD>
-hipError_t launch_plan(const PlanArgs& a, int grid, int nt, size_t lds, hipStream_t s) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&plan_kernel<MAXD>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(plan_kernel<MAXD>, dim3(grid), dim3(nt), lds, s, a);
-    return hipGetLastError();
 }
 
 template <int MAXD>
 int occupancy(int nt, size_t lds) {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, plan_kernel<MAXD>, nt, lds) != hipSuccess) return 1;
-    return blocks > 0 ? blocks : 1;
+    int b1 = 0, b2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, plan_pass1<MAXD>, nt, lds) != hipSuccess) b1 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, plan_pass2<MAXD>, nt, lds) != hipSuccess) b2 = 1;
+    return std::max(1, std::min(b1, b2));
+}
+
+template <int MAXD>
+hipError_t launch_passes(PlanArgs& a, int nt, size_t lds, hipStream_t s) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&plan_pass1<MAXD>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&plan_pass2<MAXD>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(plan_pass1<MAXD>, dim3(std::min(a.slots, a.B)), dim3(nt), lds, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    void* args[] = {&a};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&plan_pass2<MAXD>), dim3(a.slots), dim3(nt), args,
+                                      static_cast<unsigned>(lds), s);
 }
 
 int planner_impl(hipStream_t stream, int B, const float* starts, const float* goals, int gmax, const int* goal_counts,
@@ -754,7 +930,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     if (rc) return rc;
     if (world_per_query && !world->counts) return fail_with(PRRTC_E_INVALID, "per-query world needs counts");
     const int dim = at->dim, S = at->num_new_samples, ncap = at->max_nodes;
-    const WsLayout wl = ws_layout(B, dim, ncap, S);
+    const WsLayout wl = ws_layout(B, ncap, S);
     if (!workspace || workspace_bytes < wl.total) return fail_with(PRRTC_E_WORKSPACE, "workspace too small");
 
     int dev = 0;
@@ -779,8 +955,11 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     a.path = out->path; a.path_len = out->path_len; a.path_cap = out->path_capacity;
     char* ws = static_cast<char*>(workspace);
     a.radii = reinterpret_cast<float*>(ws + wl.radii);
-    a.xbuf = reinterpret_cast<float*>(ws + wl.xbuf);
-    a.queue = reinterpret_cast<int*>(ws + wl.queue);
+    a.xg = reinterpret_cast<int*>(ws + wl.xg);
+    a.qstate = reinterpret_cast<int*>(ws + wl.qstate);
+    a.ctl = reinterpret_cast<int*>(ws + wl.ctl);
+    a.cont = reinterpret_cast<int*>(ws + wl.cont);
+    a.tsync = reinterpret_cast<unsigned*>(ws + wl.tsync);
     a.stats = reinterpret_cast<unsigned long long*>(ws + wl.stats);
     a.wall_khz = di.wall_khz;
 
@@ -788,7 +967,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     for (;;) {
         a.ml = model_layout(robot->n_joints, robot->n_robot_spheres, world->n_self_pairs, world->n_world_spheres,
                             world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt / kWave, kFBP);
-        lds = planner_lds(a.ml, S, &a.off_expl, &a.off_primes, &a.off_nn, &a.off_hash, &a.off_misc, &a.hash_size);
+        lds = planner_lds(a.ml, S, dim, a);
         if (lds <= static_cast<size_t>(di.max_lds) || nt <= kWave) break;
         nt -= kWave;
     }
@@ -796,13 +975,16 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
         return fail_with(PRRTC_E_LDS, "robot + world + explorer state exceed the work-group LDS (" +
                                            std::to_string(lds) + " > " + std::to_string(di.max_lds) + " bytes)");
     const bool small = dim <= 8;
-    int slots = (launch && launch->slots > 0) ? launch->slots
-                                               : (small ? occupancy<8>(nt, lds) : occupancy<16>(nt, lds)) * di.cus;
-    slots = std::max(1, std::min(slots, B));
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, a.queue, a.stats);
+    const int resident = (small ? occupancy<8>(nt, lds) : occupancy<16>(nt, lds)) * di.cus;
+    const int slots = (launch && launch->slots > 0) ? std::min(launch->slots, resident) : resident;
+    a.slots = std::max(1, std::min(slots, kMaxSlots));
+    // A lone query goes straight to a team; a batch first gives every query a short solo run.
+    a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (B == 1 ? 0 : 24);
+    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 32;
+    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
-    e = small ? launch_plan<8>(a, slots, nt, lds, stream) : launch_plan<16>(a, slots, nt, lds, stream);
+    e = small ? launch_passes<8>(a, nt, lds, stream) : launch_passes<16>(a, nt, lds, stream);
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
     return PRRTC_OK;
 }
@@ -815,7 +997,12 @@ extern "C" {
 
 size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples) {
     if (batch < 1 || dim < 1 || max_nodes < 1 || num_new_samples < 1) return 0;
-    return ws_layout(batch, dim, max_nodes, num_new_samples).total;
+    return ws_layout(batch, max_nodes, num_new_samples).total;
+}
+
+size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples) {
+    if (batch < 1 || dim < 1 || max_nodes < 1 || num_new_samples < 1) return 0;
+    return ws_layout(batch, max_nodes, num_new_samples).stats;
 }
 
 int prrtc_planner(prrtc_stream_t stream, const float* start, const float* goals, int32_t num_goals,
@@ -836,14 +1023,5 @@ int prrtc_planner_batch(prrtc_stream_t stream, int32_t batch, const float* start
                         max_vals, robot, world, world_per_query, max_nodes_q, seeds, order, attrs, out, launch,
                         workspace, workspace_bytes);
 }
-
-size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples) {
-    if (batch < 1 || dim < 1 || max_nodes < 1 || num_new_samples < 1) return 0;
-    return ws_layout(batch, dim, max_nodes, num_new_samples).stats;
-}
-
-const char* prrtc_last_error(void) { return g_err.c_str(); }
-
-const char* prrtc_version(void) { return "prrtc_amd 0.1.0 (gfx950)"; }
 
 }  // extern "C"

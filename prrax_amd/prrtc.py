@@ -226,7 +226,7 @@ class PreparedBatch:
         L = _lib.lib()
         ptr = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         out = self.bufs.outputs()
-        launch = self.launch if self.launch is not None else _lib.Launch(0, 0)
+        launch = self.launch if self.launch is not None else _lib.Launch(0, 0, -1, 0)
         rc = L.prrtc_planner_batch(
             _stream_ptr(), self.batch, self.starts.data_ptr(), self.goals.data_ptr(), int(self.goals.shape[1]),
             ptr(self.goal_counts), self.min_vals.data_ptr(), self.max_vals.data_ptr(), self.robot.struct,
@@ -311,11 +311,11 @@ def prrtc_plan(
         out = bufs.outputs()
         rc = L.prrtc_planner(_stream_ptr(), starts_t.data_ptr(), goals_t.data_ptr(), int(goals.shape[0]),
                              lo.data_ptr(), hi.data_ptr(), robot.struct, world.struct, attrs, out,
-                             _lib.Launch(int(threads), 0), bufs.workspace.data_ptr(), bufs.workspace.numel())
+                             _lib.Launch(int(threads), 0, -1, 0), bufs.workspace.data_ptr(), bufs.workspace.numel())
         _lib.check("prrtc_planner", rc)
     else:
         PreparedBatch(robot, world, starts_t, goals_t, None, lo, hi, None, seeds_t, None, attrs, bufs,
-                      _lib.Launch(int(threads), 0)).enqueue()
+                      _lib.Launch(int(threads), 0, -1, 0)).enqueue()
     return bufs.results()[0]
 
 
@@ -346,6 +346,8 @@ def prepare_batch(
     path_capacity: int = DEFAULT_PATH_CAPACITY,
     threads: int = 0,
     slots: int = 0,
+    pass1_rounds: int = -1,
+    team_max: int = 0,
 ) -> PreparedBatch:
     """Validates and uploads a batch; ``PreparedBatch.enqueue()`` then runs it (repeatably)."""
     collision_mode = _resolve_collision_mode(collision_checker)
@@ -437,7 +439,7 @@ def prepare_batch(
         robot, world, torch.from_numpy(np.ascontiguousarray(starts)).to(dev),
         torch.from_numpy(np.ascontiguousarray(goals)).to(dev),
         None if gcounts is None else torch.from_numpy(gcounts).to(dev), lo, hi, mnq, sd, od, attrs, bufs,
-        _lib.Launch(int(threads), int(slots)))
+        _lib.Launch(int(threads), int(slots), int(pass1_rounds), int(team_max)))
 
 
 def prrtc_plan_batch(
@@ -468,6 +470,8 @@ def prrtc_plan_batch(
     path_capacity: int = DEFAULT_PATH_CAPACITY,
     threads: int = 0,
     slots: int = 0,
+    pass1_rounds: int = -1,
+    team_max: int = 0,
 ) -> list[PRRTCResult]:
     """Plan a batch of independent queries in one persistent launch (prrtc.py:882-1434).
 
@@ -482,7 +486,7 @@ def prrtc_plan_batch(
         tree_ratio=tree_ratio, dynamic_domain=dynamic_domain, dd_alpha=dd_alpha, dd_radius=dd_radius,
         dd_min_radius=dd_min_radius, min_vals=min_vals, max_vals=max_vals, collision_checker=collision_checker,
         collision_margin=collision_margin, max_nodes_per_query=max_nodes_per_query, seeds=seeds, order=order,
-        path_capacity=path_capacity, threads=threads, slots=slots)
+        path_capacity=path_capacity, threads=threads, slots=slots, pass1_rounds=pass1_rounds, team_max=team_max)
     pb.enqueue()
     return pb.bufs.results()
 
@@ -501,7 +505,7 @@ class CollisionChecker:
         self.device = dev
         self.robot = upload_robot(self.arrays, dev)
         self.world = upload_world(self.arrays, dev)
-        self.launch = _lib.Launch(int(threads), 0)
+        self.launch = _lib.Launch(int(threads), 0, -1, 0)
 
     def _configs(self, configs) -> torch.Tensor:
         if isinstance(configs, torch.Tensor) and configs.is_cuda:

@@ -214,3 +214,33 @@ def test_paths_valid_under_fp64_checker():
             for t in np.linspace(0.0, 1.0, 17):  # the planner's waypoints t = k/16
                 d = O.min_distance64(cc, a + t * (b - a))
                 assert d > -1e-5, f"query {i}: path point in collision (d={d})"
+
+
+@pytest.mark.parametrize("pass1_rounds,team_max", [(0, 1), (0, 2), (0, 8), (1, 4), (2, 32), (-1, 0)])
+def test_team_schedules_bit_exact(pass1_rounds, team_max):
+    """Hand-over budget and team size change who computes what, never the result."""
+    robot = scenes.make_arm(0, 64, 200)
+    qids = [0, 1, 2, 26, 455]  # 26 needs 158 rounds, 455 never connects within the cap
+    ctxs, starts, goals, nodes = [], [], [], []
+    for qi in qids:
+        w, ng, N = scenes.c3_query_world(robot, qi)
+        cc = dict(robot.ctx)
+        cc.update(w)
+        s, g = query(scenes.Scene(robot, w), 3, qi, n_goals=ng)
+        ctxs.append(cc); starts.append(s); goals.append(g); nodes.append(N)
+    kw = dict(num_new_samples=128, granularity=16, max_iterations=40, min_vals=robot.lower, max_vals=robot.upper)
+    say(f"team schedule pass1_rounds={pass1_rounds} team_max={team_max}")
+    res = prrtc_plan_batch(np.stack(starts), goals, collision_context=ctxs, max_nodes=max(nodes),
+                           max_nodes_per_query=nodes, pass1_rounds=pass1_rounds, team_max=team_max, **kw)
+    for k, qi in enumerate(qids):
+        ora = _oracle_cache(qi, ctxs[k], starts[k], goals[k], nodes[k], kw)
+        compare_plan(res[k], ora, f"c3 q{qi}")
+
+
+_ORA = {}
+
+
+def _oracle_cache(qi, cc, s, g, N, kw):
+    if qi not in _ORA:
+        _ORA[qi] = O.plan(O.Context(cc, 7), s, g, max_nodes=N, **kw)
+    return _ORA[qi]
