@@ -124,10 +124,12 @@ typedef struct {
 /* Workspace bytes a planner call needs (radii + per-query explorer state + queue counters). */
 size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
 
-/* Byte offset, inside that workspace, of four uint64 counters the last planner call left there:
+/* Byte offset, inside that workspace, of 16 uint64 counters the last planner call left there:
  * [0] configurations FK+collision-checked, [1] lockstep rounds summed over queries,
- * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] reserved.  No reference
- * counterpart (the reference reports only kernel_time_ms). */
+ * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] reserved,
+ * [4..9] / [10..15] device wall-clock ticks spent in the round phases of the per-query pass /
+ * the team pass (sample+NN, extend check, commit, connect NN, connect walk, connect commit).
+ * No reference counterpart (the reference reports only kernel_time_ms). */
 size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
 
 /* Single query.  Replaces PrrtcPlannerFfi (prrtc_planner.cu:2063-2117, handler 1069-1396):

@@ -313,6 +313,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     __syncthreads();
 
     unsigned long long evals = 0, flops = 0;  // lane 0 of each wave counts
+    // Per-phase wall time (member 0, thread 0): up to B1, B3, B4, B5, B7, B8.
+    unsigned long long ph[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long tph = wall_clock64();
+    auto mark = [&](int k) {
+        const unsigned long long now = wall_clock64();
+        ph[k] += now - tph;
+        tph = now;
+    };
     const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
     int r = r_begin;
     for (;; ++r) {
@@ -374,6 +382,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         team_barrier(team);  // B1: nearest neighbours of all explorers are in global memory
+        mark(0);
         // ---- phase 1c: dynamic-domain skip and steer, every explorer (redundant per member)
         for (int e = tid; e < S; e += nt) {
             const int t = side[e];
@@ -431,6 +440,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         team_barrier(team);  // B3: every extend verdict is in global memory
+        mark(1);
         // ---- phase 3: commit -- appends in explorer order (prefix sums), radius updates
         {
             for (int e = tid; e < S; e += nt)
@@ -491,6 +501,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         team_barrier(team);  // B4: new nodes and radii are in global memory
+        mark(2);
         if (sc[SC_STATUS] != 0) break;
         // ---- phase 4: connect nearest neighbour in the other tree
         {
@@ -515,6 +526,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         team_barrier(team);  // B5: connect nearest neighbours are in global memory
+        mark(3);
         for (int e = tid; e < S; e += nt) {
             if (newi[e] < 0) continue;
             const float* O = side[e] ? TA : TB;
@@ -600,6 +612,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             __syncthreads();
         }
         team_barrier(team);  // B7: every connect walk's length is in global memory
+        mark(4);
         // ---- phase 6: connect appends (explorer-major); first success or overflow decides
         {
             for (int e = tid; e < S; e += nt) nok[e] = (newi[e] >= 0) ? gload(&V.g_fail[e]) : 0;
@@ -660,6 +673,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         team_barrier(team);  // B8: connect nodes are in global memory
+        mark(5);
         if (W > 1) {
             if (tid == 0) sc[SC_ERR] = gload(team.error);
             __syncthreads();
@@ -670,6 +684,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     if (lane == 0 && evals) {
         global_add(&a.stats[0], evals);
         global_add(&a.stats[2], flops);
+    }
+    if (m == 0 && tid == 0) {
+        const int base = (r_budget >= 0) ? 4 : 10;  // pass 1 / pass 2 phase times
+        for (int k = 0; k < 6; ++k) global_add(&a.stats[base + k], ph[k]);
     }
     const int st = sc[SC_ERR] ? -1 : sc[SC_STATUS];
     const unsigned long long t1 = wall_clock64();
@@ -840,7 +858,7 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
 __global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, int nsync) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < kCtl) ctl[t] = 0;
-    if (t < 4) stats[t] = 0ull;
+    if (t < 16) stats[t] = 0ull;
     for (int i = t; i < 2 * nsync; i += gridDim.x * blockDim.x) tsync[i] = 0u;
 }
 
@@ -860,7 +878,7 @@ WsLayout ws_layout(int B, int ncap, int S) {
     w.ctl = o; o = align16(o + sizeof(int) * kCtl);
     w.cont = o; o = align16(o + sizeof(int) * static_cast<size_t>(B));
     w.tsync = o; o = align16(o + sizeof(unsigned) * 2 * kMaxSlots);
-    w.stats = o; o = align16(o + 4 * sizeof(unsigned long long));
+    w.stats = o; o = align16(o + 16 * sizeof(unsigned long long));
     w.total = o + 256;
     return w;
 }

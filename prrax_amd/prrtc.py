@@ -124,10 +124,16 @@ class PlanBuffers:
         self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
         self.stats_offset = int(_lib.lib().prrtc_planner_stats_offset(batch, dim, max_nodes, num_new_samples))
 
+    PHASES = ("sample_nn", "extend_check", "commit", "connect_nn", "connect_walk", "connect_commit")
+
     def stats(self) -> dict:
-        """Counters of the last launch: configs checked, rounds, algorithmic flops."""
-        raw = self.workspace[self.stats_offset:self.stats_offset + 32].cpu().numpy().view(np.uint64)
-        return {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2])}
+        """Counters of the last launch: configs checked, rounds, algorithmic flops, and device
+        wall-clock ticks per round phase for the per-query pass and the team pass."""
+        raw = self.workspace[self.stats_offset:self.stats_offset + 128].cpu().numpy().view(np.uint64)
+        out = {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2])}
+        out["pass1_ticks"] = {k: int(v) for k, v in zip(self.PHASES, raw[4:10])}
+        out["pass2_ticks"] = {k: int(v) for k, v in zip(self.PHASES, raw[10:16])}
+        return out
 
     def outputs(self) -> _lib.Outputs:
         B = self.batch
