@@ -40,10 +40,8 @@
 
 namespace prrtc {
 
-constexpr int kPassMult = 2;  // connect items per pass ~= kPassMult * (waves * kFB)
-
 // Per-query explorer arrays kept in global memory ([B][XG_COUNT][S] ints).
-enum { XG_NN, XG_ONN, XG_DSQ, XG_FAIL, XG_VAMP, XG_COUNT };
+enum { XG_NN, XG_ONN, XG_DSQ, XG_FAIL, XG_VAMP, XG_ECLAIM, XG_CLAIM, XG_CFAIL, XG_COUNT };
 // Per-query round state ([B][QS_COUNT] ints) for the pass-1 -> pass-2 hand-over.
 enum { QS_ROUND, QS_SIZE_A, QS_SIZE_B, QS_TIME, QS_COUNT = 8 };
 // Control words ([kCtl] ints).
@@ -105,7 +103,7 @@ struct PlanArgs {
 };
 
 // Explorer arrays in each member's LDS (S + 1 ints each).
-enum { X_SIDE, X_NN, X_STATE, X_NEWI, X_ONN, X_NEXT, X_NOK, X_KD, X_VAMP, X_LIST, X_OFFS, X_AUX0, X_AUX1, X_COUNT };
+enum { X_SIDE, X_NN, X_STATE, X_NEWI, X_ONN, X_NEXT, X_NOK, X_VAMP, X_LIST, X_AUX0, X_AUX1, X_COUNT };
 
 __host__ __device__ inline size_t expl_bytes(int S) { return align16(sizeof(int) * (size_t)(S + 1) * X_COUNT); }
 __host__ __device__ inline int hash_size_for(int S) {
@@ -247,6 +245,7 @@ struct QueryView {
     float *TA, *TB, *RA, *RB;
     int *PA, *PB;
     int *g_nn, *g_onn, *g_fail, *g_vamp;
+    int *g_eclaim, *g_claim, *g_cfail;  // extend / connect claim counters, first failing sub-edge
     float* g_dsq;
     int* qs;
 };
@@ -265,10 +264,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     int* onn = X + X_ONN * (S + 1);
     int* next = X + X_NEXT * (S + 1);
     int* nok = X + X_NOK * (S + 1);
-    int* kd = X + X_KD * (S + 1);
     int* vamp = X + X_VAMP * (S + 1);
     int* list = X + X_LIST * (S + 1);
-    int* offs = X + X_OFFS * (S + 1);
     int* aux0 = X + X_AUX0 * (S + 1);
     int* aux1 = X + X_AUX1 * (S + 1);
     float* cand = reinterpret_cast<float*>(lds + a.off_cand);
@@ -357,7 +354,12 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 t = (size_a <= size_b) ? 0 : 1;
             }
             side[e] = t;
-            if (e % W == m) gstore(&V.g_fail[e], 0);
+            if (e % W == m) {
+                gstore(&V.g_fail[e], 0);
+                gstore(&V.g_eclaim[e], 0);
+                gstore(&V.g_claim[e], 0);
+                gstore(&V.g_cfail[e], 0x7fffffff);
+            }
         }
         __syncthreads();
         // ---- phase 1b: nearest neighbour of each explorer's sample in its tree; tiles of kEC
@@ -404,37 +406,51 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         __syncthreads();
-        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Each wave takes kFB
-        // (explorer, waypoint) items, computes their FK one per lane, then checks them one by one;
-        // the team's waves share the item batches.
+        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = kFB
+        // consecutive waypoints of one edge; any wave of the team claims units of any explorer
+        // whose edge is not yet known to collide (FK one per lane, then checks in order).
         {
-            const int nv = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
-            const int items = nv * G;
-            const int nb = (items + kFB - 1) / kFB;
-            for (int b = m * nwaves + wid; b < nb; b += W * nwaves) {
-                const int it = b * kFB + lane;
+            const int cps = (G + kFB - 1) / kFB;
+            int start = ((m * nwaves + wid) * S) / (W * nwaves);
+            for (;;) {
                 int e = -1;
-                if (lane < kFB && it < items) {
-                    const int slot = it / G, k = it - slot * G;
-                    e = list[slot];
-                    if (gload(&V.g_fail[e])) {
-                        e = -1;
-                    } else {
-                        const float* T = side[e] ? TB : TA;
-                        const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
-                        for (int d = 0; d < dim; ++d) {
-                            const float base = T[static_cast<size_t>(d) * ncap + nn[e]];
-                            L.q[d * kFBP] = base + tf * (cand[e * dim + d] - base);
-                        }
-                        if (M.enabled) fk_lane(M, L);
+                for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
+                    bool ok = false;
+                    int ex = 0;
+                    if (b0 + lane < S) {
+                        ex = start + b0 + lane;
+                        if (ex >= S) ex -= S;
+                        ok = state[ex] == 1 && gload(&V.g_fail[ex]) == 0 && gload(&V.g_eclaim[ex]) < cps;
                     }
+                    const unsigned long long mask = __ballot(ok);
+                    if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                }
+                e = uniform(e);
+                if (e < 0) break;
+                start = e;
+                int u = 0;
+                if (lane == 0) u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_eclaim[e], 1, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                u = uniform(__shfl(u, 0, kWave));
+                if (u >= cps) continue;  // another wave took the last unit
+                const int k0 = u * kFB, nk = min(kFB, G - k0);
+                if (lane < nk) {
+                    const float* T = side[e] ? TB : TA;
+                    const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
+                    for (int d = 0; d < dim; ++d) {
+                        const float base = T[static_cast<size_t>(d) * ncap + nn[e]];
+                        L.q[d * kFBP] = base + tf * (cand[e * dim + d] - base);
+                    }
+                    if (M.enabled) fk_lane(M, L);
                 }
                 wave_sync();
-                for (int c = 0; c < kFB; ++c) {
-                    const int ec = uniform(__shfl(e, c, kWave));
-                    if (ec < 0 || uniform(gload(&V.g_fail[ec]))) continue;
+                for (int c = 0; c < nk; ++c) {
+                    if (uniform(gload(&V.g_fail[e]))) break;
                     if (lane == 0) ++evals;
-                    if (wave_in_collision(M, WA.T + c, kFBP, flops) && lane == 0) gstore(&V.g_fail[ec], 1);
+                    if (wave_in_collision(M, WA.T + c, kFBP, flops)) {
+                        if (lane == 0) gstore(&V.g_fail[e], 1);
+                        break;
+                    }
                 }
                 wave_sync();
             }
@@ -536,86 +552,71 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             int n_ext = 0;
             if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
             next[e] = n_ext;
-            kd[e] = 0;
             for (int d = 0; d < dim; ++d)
                 vec[e * dim + d] = (n_ext > 0)
                     ? (O[static_cast<size_t>(d) * ncap + oi] - cand[e * dim + d]) / static_cast<float>(n_ext)
                     : 0.0f;
-            if (e % W == m) gstore(&V.g_fail[e], n_ext);
         }
         __syncthreads();
-        // ---- phase 5: each member walks the connect sub-edges of its own explorers in passes;
-        // g_fail[e] ends as the first failing sub-edge (or n_ext).
-        for (;;) {
-            const int na = compact(S, [&](int e) {
-                return e % W == m && newi[e] >= 0 && kd[e] < next[e] && gload(&V.g_fail[e]) == next[e];
-            }, list, scan);
-            if (na == 0) break;
-            const int mper = max(1, (nwaves * kFB * kPassMult) / (na * G));
-            int run = 0;
-            for (int s0 = 0; s0 < na; s0 += nt) {
-                const int s = s0 + tid;
-                int v = 0;
-                if (s < na) {
-                    const int e = list[s];
-                    v = min(mper, next[e] - kd[e]) * G;
-                }
-                int tot;
-                const int pre = block_excl_scan(v, scan, &tot);
-                if (s < na) offs[s] = run + pre;
-                run += tot;
-            }
-            if (tid == 0) offs[na] = run;
-            __syncthreads();
-            const int nb = (run + kFB - 1) / kFB;
-            for (int b = wid; b < nb; b += nwaves) {
-                const int it = b * kFB + lane;
-                int e = -1, j = 0;
-                if (lane < kFB && it < run) {
-                    int lo = 0, hi = na;  // offs[lo] <= it < offs[hi]
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (offs[mid] <= it) lo = mid; else hi = mid;
-                    }
-                    e = list[lo];
-                    const int local = it - offs[lo];
-                    j = kd[e] + local / G;
-                    const int k = local - (local / G) * G;
-                    if (j >= gload(&V.g_fail[e])) {
-                        e = -1;
-                    } else {
-                        const float tf = static_cast<float>(k + 1) / static_cast<float>(G);
-                        for (int d = 0; d < dim; ++d) {
-                            float c = cand[e * dim + d];
-                            const float v = vec[e * dim + d];
-                            for (int jj = 0; jj < j; ++jj) c += v;
-                            L.q[d * kFBP] = c + tf * v;
+        // ---- phase 5: connect walks.  Work unit = kFB consecutive waypoints of sub-edge j of one
+        // explorer's walk; waves claim units of any explorer (claim counter per explorer), and a
+        // collision at sub-edge j lowers the explorer's cap to j, so units past the first failing
+        // sub-edge stop being handed out.  g_cfail[e] ends as that first failing sub-edge.
+        {
+            const int cps = (G + kFB - 1) / kFB;
+            int start = ((m * nwaves + wid) * S) / (W * nwaves);
+            for (;;) {
+                int e = -1;
+                for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
+                    bool ok = false;
+                    int ex = 0;
+                    if (b0 + lane < S) {
+                        ex = start + b0 + lane;
+                        if (ex >= S) ex -= S;
+                        if (newi[ex] >= 0) {
+                            const int cap = min(next[ex], gload(&V.g_cfail[ex])) * cps;
+                            ok = gload(&V.g_claim[ex]) < cap;
                         }
-                        if (M.enabled) fk_lane(M, L);
+                    }
+                    const unsigned long long mask = __ballot(ok);
+                    if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                }
+                e = uniform(e);
+                if (e < 0) break;
+                start = e;
+                int u = 0;
+                if (lane == 0) u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_claim[e], 1, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                u = uniform(__shfl(u, 0, kWave));
+                const int j = u / cps, k0 = (u - j * cps) * kFB, nk = min(kFB, G - k0);
+                if (j >= min(next[e], uniform(gload(&V.g_cfail[e])))) continue;  // past the end or the first failure
+                if (lane < nk) {
+                    const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
+                    for (int d = 0; d < dim; ++d) {
+                        float c = cand[e * dim + d];
+                        const float v = vec[e * dim + d];
+                        for (int jj = 0; jj < j; ++jj) c += v;
+                        L.q[d * kFBP] = c + tf * v;
+                    }
+                    if (M.enabled) fk_lane(M, L);
+                }
+                wave_sync();
+                for (int c = 0; c < nk; ++c) {
+                    if (j >= uniform(gload(&V.g_cfail[e]))) break;
+                    if (lane == 0) ++evals;
+                    if (wave_in_collision(M, WA.T + c, kFBP, flops)) {
+                        if (lane == 0) gmin(&V.g_cfail[e], j);
+                        break;
                     }
                 }
                 wave_sync();
-                for (int c = 0; c < kFB; ++c) {
-                    const int ec = uniform(__shfl(e, c, kWave));
-                    const int jc = uniform(__shfl(j, c, kWave));
-                    if (ec < 0 || jc >= uniform(gload(&V.g_fail[ec]))) continue;
-                    if (lane == 0) ++evals;
-                    if (wave_in_collision(M, WA.T + c, kFBP, flops) && lane == 0) gmin(&V.g_fail[ec], jc);
-                }
-                wave_sync();
             }
-            __syncthreads();
-            for (int s = tid; s < na; s += nt) {
-                const int e = list[s];
-                kd[e] += min(mper, next[e] - kd[e]);
-            }
-            __syncthreads();
         }
         team_barrier(team);  // B7: every connect walk's length is in global memory
         mark(4);
         // ---- phase 6: connect appends (explorer-major); first success or overflow decides
         {
-            for (int e = tid; e < S; e += nt) nok[e] = (newi[e] >= 0) ? gload(&V.g_fail[e]) : 0;
+            for (int e = tid; e < S; e += nt) nok[e] = (newi[e] >= 0) ? min(next[e], gload(&V.g_cfail[e])) : 0;
             if (tid == 0) sc[SC_EVENT] = S;
             __syncthreads();
             int base0 = sc[SC_SIZE_A], base1 = sc[SC_SIZE_B];
@@ -781,6 +782,9 @@ __device__ QueryView query_view(const PlanArgs& a, int q) {
     V.g_dsq = reinterpret_cast<float*>(xg + XG_DSQ * a.S);
     V.g_fail = xg + XG_FAIL * a.S;
     V.g_vamp = xg + XG_VAMP * a.S;
+    V.g_eclaim = xg + XG_ECLAIM * a.S;
+    V.g_claim = xg + XG_CLAIM * a.S;
+    V.g_cfail = xg + XG_CFAIL * a.S;
     V.qs = a.qstate + static_cast<size_t>(q) * QS_COUNT;
     return V;
 }
