@@ -74,6 +74,19 @@ def cpu_baseline(worlds, starts, goals, nodes, kw, n_sample: int, workers: int):
             "solve_rate": round(float(np.mean(solved)), 4), "wall_s": round(dt, 2)}
 
 
+def copy_calibration(mib: int):
+    """One plain copy of ``mib`` MiB: its FETCH_SIZE / WRITE_SIZE readings calibrate the HBM counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {mib} MiB read + {mib} MiB written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,12 +101,18 @@ def main():
     ap.add_argument("--fkcc-configs", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--max-iterations", type=int, default=10_000)
+    ap.add_argument("--team-max", type=int, default=0, help="largest work-group team per open query (0 = library default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every query is independent and runs on a single device; "
-              "multi-GPU runs are not part of this build (see DESIGN.md for how the queue would shard).")
+              "runs on several GPUs are not supported (the query queue would be split per GPU).")
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     import torch
 
@@ -117,7 +136,8 @@ def main():
     order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
                        kind="stable")
     pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
-                       max_nodes_per_query=nodes, order=order, threads=args.threads, slots=args.slots, **kw)
+                       max_nodes_per_query=nodes, order=order, threads=args.threads, slots=args.slots,
+                       team_max=args.team_max, **kw)
     # ---- batched
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -142,30 +162,34 @@ def main():
         f"solve rate {solved_b.mean():.4f}, {stats}")
     # ---- per-query launches
     n_pq = args.queries if args.per_query < 0 else min(args.per_query, args.queries)
-    singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=[worlds[i]], max_nodes=nodes[i],
-                             threads=args.threads, **kw) for i in range(n_pq)]
-    singles[0].enqueue()
-    torch.cuda.synchronize()
-    times = []
-    for k, sp in enumerate(singles):
-        t = time.perf_counter()
-        sp.enqueue()
+    pq = None
+    if n_pq > 0:
+        singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=[worlds[i]], max_nodes=nodes[i],
+                                 threads=args.threads, **kw) for i in range(n_pq)]
+        singles[0].enqueue()
         torch.cuda.synchronize()
-        times.append(time.perf_counter() - t)
-        if (k + 1) % 128 == 0:
-            log(f"per-query launches: {k + 1}/{n_pq} done, {sum(times):.2f}s")
-    solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
-    pq_plans_per_s = n_pq / sum(times)
-    same = bool(np.array_equal(solved_pq, solved_b[:n_pq]))
-    log(f"per-query: {pq_plans_per_s:.1f} plans/s, p50 {1000 * np.median(times):.3f} ms, solve rate "
-        f"{solved_pq.mean():.4f}, identical solved flags: {same}")
+        times = []
+        for k, sp in enumerate(singles):
+            t = time.perf_counter()
+            sp.enqueue()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+            if (k + 1) % 128 == 0:
+                log(f"per-query launches: {k + 1}/{n_pq} done, {sum(times):.2f}s")
+        solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
+        pq_plans_per_s = n_pq / sum(times)
+        same = bool(np.array_equal(solved_pq, solved_b[:n_pq]))
+        log(f"per-query: {pq_plans_per_s:.1f} plans/s, p50 {1000 * np.median(times):.3f} ms, solve rate "
+            f"{solved_pq.mean():.4f}, identical solved flags: {same}")
+        pq = {"plans_per_s": round(pq_plans_per_s, 2), "p50_ms_per_plan": round(1000 * float(np.median(times)), 3),
+              "solve_rate": round(float(solved_pq.mean()), 4), "queries": n_pq, "identical_solved_flags": same}
     # ---- FK + CC throughput (large-scene world, 10^6 configs)
     sc2 = scenes.scene_c2()
     rng = np.random.default_rng(11)
     qcfg = torch.from_numpy(rng.uniform(sc2.robot.lower, sc2.robot.upper,
                                         size=(args.fkcc_configs, 7)).astype(np.float32)).cuda()
     fkcc = {}
-    for mode in ("binary", "differentiable"):
+    for mode in (("binary", "differentiable") if args.fkcc_configs > 0 else ()):
         chk = CollisionChecker(sc2.context(), 7, collision_checker=mode, collision_margin=0.02)
         hit = torch.empty(args.fkcc_configs, dtype=torch.uint8, device="cuda")
         flops = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -207,10 +231,8 @@ def main():
                     "p50_ms_per_plan_device": round(float(np.median(summ["kernel_time_ms"])), 3),
                     "kernel_ms_per_step": round(kernel_ms, 3), "configs_checked": stats["configs"],
                     "rounds": stats["rounds"]},
-        "per_query": {"plans_per_s": round(pq_plans_per_s, 2), "p50_ms_per_plan": round(1000 * float(np.median(times)), 3),
-                      "solve_rate": round(float(solved_pq.mean()), 4), "queries": n_pq,
-                      "identical_solved_flags": same},
-        "batched_over_per_query": round(plans_per_s / pq_plans_per_s, 2),
+        "per_query": pq,
+        "batched_over_per_query": round(plans_per_s / pq["plans_per_s"], 2) if pq else None,
         "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
                              "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
         "roofline": {"kernel": "plan_kernel", "bound": "valu", "achieved": round(achieved, 4),
