@@ -84,7 +84,7 @@ class Outputs(ctypes.Structure):
 
 class Launch(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("slots", ctypes.c_int32), ("pass1_rounds", ctypes.c_int32),
-                ("team_max", ctypes.c_int32)]
+                ("team_max", ctypes.c_int32), ("epoch_rounds", ctypes.c_int32)]
 
 
 class PrrtcError(RuntimeError):

@@ -251,18 +251,18 @@ __device__ __forceinline__ bool lex_less(float da, int ia, float db, int ib) {
 }
 
 // Nearest node (lowest index on ties) of kEC queries (qsh[c * kMaxDim + d], unused rows zero)
-// among tree[0, size).  Every thread of the work-group calls it; on return red_d[c * 16] and
-// red_i[c * 16] hold query c's squared distance and index.
+// among tree[begin, end).  Every thread of the work-group calls it; on return red_d[c * 16] and
+// red_i[c * 16] hold query c's squared distance and index ((FLT_MAX, 0) for an empty range).
 template <int MAXD>
-__device__ void nn_tile(const float* __restrict__ tree, int ncap, int size, int dim, const float* qsh, float* red_d,
-                        int* red_i) {
+__device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int end, int dim, const float* qsh,
+                        float* red_d, int* red_i) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int nw = (nt + kWave - 1) / kWave;
     float bd[kEC];
     int bi[kEC];
 #pragma unroll
     for (int c = 0; c < kEC; ++c) { bd[c] = FLT_MAX; bi[c] = 0; }
-    for (int i = tid; i < size; i += nt) {
+    for (int i = begin + tid; i < end; i += nt) {
         float x[MAXD];
 #pragma unroll
         for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? tree[static_cast<size_t>(d) * ncap + i] : 0.0f;
@@ -322,7 +322,7 @@ __device__ void nn_tiles(const float* __restrict__ tree, int ncap, int size, int
             qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qbuf[list[c0 + c] * dim + d] : 0.0f;
         }
         __syncthreads();
-        nn_tile<MAXD>(tree, ncap, size, dim, qsh, red_d, red_i);
+        nn_tile<MAXD>(tree, ncap, 0, size, dim, qsh, red_d, red_i);
         if (tid < nc) {
             const int e = list[c0 + tid];
             out_i[e] = red_i[tid * 16];

@@ -40,12 +40,16 @@
 
 namespace prrtc {
 
-// Per-query explorer arrays kept in global memory ([B][XG_COUNT][S] ints).
-enum { XG_NN, XG_ONN, XG_DSQ, XG_FAIL, XG_VAMP, XG_ECLAIM, XG_CLAIM, XG_CFAIL, XG_COUNT };
+// Per-query explorer arrays kept in global memory ([B][XG_COUNT][S] ints).  XG_KEY / XG_OKEY are
+// 64-bit nearest-neighbour keys (two int slots each, first so that they stay 8-byte aligned;
+// XG_COUNT is even).
+enum { XG_KEY = 0, XG_OKEY = 2, XG_FAIL = 4, XG_VAMP, XG_ECLAIM, XG_CLAIM, XG_CFAIL, XG_PAD, XG_COUNT };
+static_assert(XG_COUNT % 2 == 0, "keys of every query must stay 8-byte aligned");
 // Per-query round state ([B][QS_COUNT] ints) for the pass-1 -> pass-2 hand-over.
 enum { QS_ROUND, QS_SIZE_A, QS_SIZE_B, QS_TIME, QS_COUNT = 8 };
-// Control words ([kCtl] ints).
-enum { CTL_QUEUE, CTL_NCONT, CTL_ERROR, kCtl = 16 };
+// Control words ([kCtl] ints).  CTL_NC + i counts continue list i (three lists rotate over the
+// epochs of pass 2: one read, one filled, one cleared).
+enum { CTL_QUEUE, CTL_ERROR, CTL_NC, kCtl = 16 };
 
 struct PlanArgs {
     int max_iterations;
@@ -88,12 +92,13 @@ struct PlanArgs {
     int* xg;                     // [B][XG_COUNT][S]
     int* qstate;                 // [B][QS_COUNT]
     int* ctl;                    // [kCtl]
-    int* cont;                   // [B]
-    unsigned* tsync;             // [slots][2] (arrivals, generation)
+    int* cont;                   // [3][B] continue lists
+    unsigned* tsync;             // [slots + 1][2] (arrivals, generation); the last pair is the grid's
     unsigned long long* stats;   // [4]
     // scheduling
     int pass1_rounds;            // round budget of pass 1 (0: pass 1 only initialises)
     int wmax;                    // team size cap
+    int epoch_rounds;            // round budget of a query per pass-2 epoch (teams re-form between epochs)
     int slots;
     // LDS layout
     ModelLayout ml;
@@ -195,6 +200,21 @@ __device__ __forceinline__ void gstore(int* p, int v) {
 __device__ __forceinline__ void gmin(int* p, int v) {
     __hip_atomic_fetch_min((PRRTC_GLOBAL int*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned long long gload64(const unsigned long long* p) {
+    return __hip_atomic_load((const PRRTC_GLOBAL unsigned long long*)(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gstore64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store((PRRTC_GLOBAL unsigned long long*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gmin64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_min((PRRTC_GLOBAL unsigned long long*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Nearest-neighbour key: squared distances are >= 0, so their bit patterns order like the values
+// and the key order is the (distance, lowest index) order.
+__device__ __forceinline__ unsigned long long nn_key(float dsq, int idx) {
+    return (static_cast<unsigned long long>(__float_as_uint(dsq)) << 32) | static_cast<unsigned>(idx);
+}
+constexpr unsigned long long kNoKey = ~0ull;
 
 // Team of W work-groups working on one query.
 struct Team {
@@ -244,15 +264,15 @@ struct QueryView {
     unsigned seed;
     float *TA, *TB, *RA, *RB;
     int *PA, *PB;
-    int *g_nn, *g_onn, *g_fail, *g_vamp;
+    unsigned long long *g_key, *g_okey;  // (squared distance bits << 32 | node) of the extend / connect NN
+    int *g_fail, *g_vamp;
     int *g_eclaim, *g_claim, *g_cfail;  // extend / connect claim counters, first failing sub-edge
-    float* g_dsq;
     int* qs;
 };
 
 template <int MAXD>
 __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team& team, int r_begin, int r_budget,
-                          unsigned long long t0) {
+                          unsigned long long t0, int pass, int* push_cnt, int* push_list) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & (kWave - 1), wid = tid / kWave, nwaves = nt / kWave;
     const int dim = a.dim, S = a.S, G = a.G, ncap = a.ncap, N = V.N, W = team.W, m = team.m;
@@ -326,7 +346,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             __syncthreads();
             break;
         }
-        if (r >= r_end) break;  // pass-1 budget spent: hand over to a team
+        if (r >= r_end) break;  // round budget spent: the query goes to the next continue list
         const int size_a = sc[SC_SIZE_A], size_b = sc[SC_SIZE_B];
         // ---- phase 1a: tree side of every explorer (each member keeps a full copy); owners clear
         // the shared collision flags of their explorers for this round.
@@ -363,10 +383,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         }
         __syncthreads();
         // ---- phase 1b: nearest neighbour of each explorer's sample in its tree; tiles of kEC
-        // explorers, tile c on member c % W
+        // explorers, each member scans its slice of the tree and merges through a key minimum
         for (int t = 0; t < 2; ++t) {
             const int nl = compact(S, [&](int e) { return side[e] == t; }, list, scan);
-            for (int c0 = m * kEC; c0 < nl; c0 += W * kEC) {
+            const int sz = t ? size_b : size_a;
+            const int lo = static_cast<int>((static_cast<long long>(sz) * m) / W);
+            const int hi = static_cast<int>((static_cast<long long>(sz) * (m + 1)) / W);
+            if (lo >= hi) continue;
+            for (int c0 = 0; c0 < nl; c0 += kEC) {
                 const int nc = min(kEC, nl - c0);
                 for (int k = tid; k < kEC * MAXD; k += nt) {
                     const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
@@ -374,12 +398,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                         ? sample_dim(a.min_vals, a.max_vals, primes + 16 * list[c0 + c], d, r) : 0.0f;
                 }
                 __syncthreads();
-                nn_tile<MAXD>(t ? TB : TA, ncap, t ? size_b : size_a, dim, qsh, red_d, red_i);
-                if (tid < nc) {
-                    const int e = list[c0 + tid];
-                    gstore(&V.g_nn[e], red_i[tid * 16]);
-                    gstore(reinterpret_cast<int*>(&V.g_dsq[e]), __float_as_int(red_d[tid * 16]));
-                }
+                nn_tile<MAXD>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
+                if (tid < nc) gmin64(&V.g_key[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
                 __syncthreads();
             }
         }
@@ -390,9 +410,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const int t = side[e];
             const float* T = t ? TB : TA;
             const float* R = t ? RB : RA;
-            const int ni = gload(&V.g_nn[e]);
+            const unsigned long long key = gload64(&V.g_key[e]);
+            const int ni = static_cast<int>(key & 0xffffffffull);
             nn[e] = ni;
-            const float nd = __builtin_sqrtf(__int_as_float(gload(reinterpret_cast<const int*>(&V.g_dsq[e]))));
+            const float nd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
             if (a.dd && R[ni] < nd) {
                 state[e] = 0;
             } else {
@@ -459,8 +480,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         mark(1);
         // ---- phase 3: commit -- appends in explorer order (prefix sums), radius updates
         {
-            for (int e = tid; e < S; e += nt)
+            for (int e = tid; e < S; e += nt) {
                 if (state[e] == 1 && gload(&V.g_fail[e])) state[e] = 3;  // 3: collided
+                if (e % W == m) gstore64(&V.g_key[e], kNoKey);  // read by every member before B3
+            }
             if (tid == 0) sc[SC_OVERFLOW] = 0;
             __syncthreads();
             int base0 = size_a, base1 = size_b;
@@ -524,19 +547,19 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const int sa = sc[SC_SIZE_A], sb = sc[SC_SIZE_B];
             for (int t = 0; t < 2; ++t) {  // t: tree searched (the other tree of the extended one)
                 const int nl = compact(S, [&](int e) { return newi[e] >= 0 && side[e] == 1 - t; }, list, scan);
-                for (int c0 = m * kEC; c0 < nl; c0 += W * kEC) {
+                const int sz = t ? sb : sa;
+                const int lo = static_cast<int>((static_cast<long long>(sz) * m) / W);
+                const int hi = static_cast<int>((static_cast<long long>(sz) * (m + 1)) / W);
+                if (lo >= hi) continue;
+                for (int c0 = 0; c0 < nl; c0 += kEC) {
                     const int nc = min(kEC, nl - c0);
                     for (int k = tid; k < kEC * MAXD; k += nt) {
                         const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
                         qsh[c * kMaxDim + d] = (c < nc && d < dim) ? cand[list[c0 + c] * dim + d] : 0.0f;
                     }
                     __syncthreads();
-                    nn_tile<MAXD>(t ? TB : TA, ncap, t ? sb : sa, dim, qsh, red_d, red_i);
-                    if (tid < nc) {
-                        const int e = list[c0 + tid];
-                        gstore(&V.g_onn[e], red_i[tid * 16]);
-                        gstore(reinterpret_cast<int*>(&V.g_dsq[e]), __float_as_int(red_d[tid * 16]));
-                    }
+                    nn_tile<MAXD>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
+                    if (tid < nc) gmin64(&V.g_okey[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
                     __syncthreads();
                 }
             }
@@ -546,9 +569,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         for (int e = tid; e < S; e += nt) {
             if (newi[e] < 0) continue;
             const float* O = side[e] ? TA : TB;
-            const int oi = gload(&V.g_onn[e]);
+            const unsigned long long key = gload64(&V.g_okey[e]);
+            const int oi = static_cast<int>(key & 0xffffffffull);
             onn[e] = oi;
-            const float cd = __builtin_sqrtf(__int_as_float(gload(reinterpret_cast<const int*>(&V.g_dsq[e]))));
+            const float cd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
             int n_ext = 0;
             if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
             next[e] = n_ext;
@@ -616,7 +640,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         mark(4);
         // ---- phase 6: connect appends (explorer-major); first success or overflow decides
         {
-            for (int e = tid; e < S; e += nt) nok[e] = (newi[e] >= 0) ? min(next[e], gload(&V.g_cfail[e])) : 0;
+            for (int e = tid; e < S; e += nt) {
+                nok[e] = (newi[e] >= 0) ? min(next[e], gload(&V.g_cfail[e])) : 0;
+                if (e % W == m) gstore64(&V.g_okey[e], kNoKey);  // read by every member before B7
+            }
             if (tid == 0) sc[SC_EVENT] = S;
             __syncthreads();
             int base0 = sc[SC_SIZE_A], base1 = sc[SC_SIZE_B];
@@ -687,14 +714,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         global_add(&a.stats[2], flops);
     }
     if (m == 0 && tid == 0) {
-        const int base = (r_budget >= 0) ? 4 : 10;  // pass 1 / pass 2 phase times
+        const int base = pass == 1 ? 4 : 10;  // pass 1 / pass 2 phase times
         for (int k = 0; k < 6; ++k) global_add(&a.stats[base + k], ph[k]);
     }
     const int st = sc[SC_ERR] ? -1 : sc[SC_STATUS];
     const unsigned long long t1 = wall_clock64();
     const float dt = static_cast<float>(static_cast<double>(t1 - t0) / a.wall_khz);
     if (st == 0) {
-        // pass-1 budget spent: save the round state (every member holds the same values)
+        // round budget spent: save the round state (every member holds the same values)
         if (m == 0) {
             for (int e = tid; e < S; e += nt) V.g_vamp[e] = vamp[e];
             if (tid == 0) {
@@ -702,8 +729,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 V.qs[QS_SIZE_A] = sc[SC_SIZE_A];
                 V.qs[QS_SIZE_B] = sc[SC_SIZE_B];
                 V.qs[QS_TIME] = __float_as_int(__int_as_float(V.qs[QS_TIME]) + dt);
-                const int k = global_add(&a.ctl[CTL_NCONT], 1);
-                a.cont[k] = V.q;
+                const int k = global_add(push_cnt, 1);
+                push_list[k] = V.q;
             }
         }
         __syncthreads();
@@ -777,9 +804,8 @@ __device__ QueryView query_view(const PlanArgs& a, int q) {
     V.RA = a.radii + static_cast<size_t>(q) * 2 * a.ncap;
     V.RB = V.RA + a.ncap;
     int* xg = a.xg + static_cast<size_t>(q) * XG_COUNT * a.S;
-    V.g_nn = xg + XG_NN * a.S;
-    V.g_onn = xg + XG_ONN * a.S;
-    V.g_dsq = reinterpret_cast<float*>(xg + XG_DSQ * a.S);
+    V.g_key = reinterpret_cast<unsigned long long*>(xg + XG_KEY * a.S);
+    V.g_okey = reinterpret_cast<unsigned long long*>(xg + XG_OKEY * a.S);
     V.g_fail = xg + XG_FAIL * a.S;
     V.g_vamp = xg + XG_VAMP * a.S;
     V.g_eclaim = xg + XG_ECLAIM * a.S;
@@ -808,7 +834,11 @@ __device__ void init_query(const PlanArgs& a, const QueryView& V) {
         V.qs[QS_TIME] = __float_as_int(0.0f);
     }
     for (int g = tid; g < V.ng; g += nt) { V.PB[g] = g; V.RB[g] = FLT_MAX; }
-    for (int e = tid; e < a.S; e += nt) V.g_vamp[e] = 0;
+    for (int e = tid; e < a.S; e += nt) {
+        V.g_vamp[e] = 0;
+        V.g_key[e] = kNoKey;
+        V.g_okey[e] = kNoKey;
+    }
     __syncthreads();
 }
 
@@ -827,35 +857,54 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
         const unsigned long long t0 = wall_clock64();
         const QueryView V = query_view(a, a.order ? a.order[qi] : qi);
         init_query(a, V);
-        run_query<MAXD>(a, V, lds, team, 1, a.pass1_rounds, t0);
+        run_query<MAXD>(a, V, lds, team, 1, a.pass1_rounds, t0, 1, a.ctl + CTL_NC, a.cont);
     }
 }
 
-// Pass 2: teams of W work-groups resume the queries left open by pass 1.
+// Pass 2: teams of W work-groups resume the queries left open by pass 1, in epochs.  In each
+// epoch the open queries are dealt to teams (W = slots / open queries, a power of two, capped)
+// and run for at most `epoch_rounds` rounds; the queries still open then form the next epoch's
+// list, so the teams grow as queries finish.  A grid barrier separates the epochs.
 template <int MAXD>
 __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
-    const int ncont = a.ctl[CTL_NCONT];
-    if (ncont == 0) return;
-    int W = 1;
-    while (W * 2 <= a.wmax && W * 2 * ncont <= a.slots) W *= 2;
+    __shared__ int s_ncont;
     const int b = blockIdx.x;
-    int team_id, member;
-    if (a.slots % (8 * W) == 0) {  // keep a team on one XCD (work-groups are dealt round-robin)
-        team_id = (b % 8) + 8 * ((b / 8) / W);
-        member = (b / 8) % W;
-    } else {
-        team_id = b / W;
-        member = b % W;
-    }
-    const int nteams = a.slots / W;
-    if (team_id >= nteams) return;
-    Team team{W, member, a.tsync + 2 * team_id, a.ctl + CTL_ERROR, 0u};
-    for (int k = team_id; k < ncont; k += nteams) {
-        const unsigned long long t0 = wall_clock64();
-        const QueryView V = query_view(a, a.cont[k]);
-        run_query<MAXD>(a, V, lds, team, V.qs[QS_ROUND], -1, t0);
-        team_barrier(team);  // members leave the query together before the next one
+    Team grid{a.slots, b, a.tsync + 2 * a.slots, a.ctl + CTL_ERROR, 0u};
+    for (int ep = 0;; ++ep) {
+        const int cur = ep % 3, nxt = (ep + 1) % 3;
+        if (threadIdx.x == 0) s_ncont = gload(&a.ctl[CTL_NC + cur]);
+        __syncthreads();
+        const int ncont = s_ncont;
+        if (ncont == 0) return;  // every work-group reads the same count after the grid barrier
+        // list (ep + 2) % 3 was read in the previous epoch and is filled in the next one
+        if (b == 0 && threadIdx.x == 0) gstore(&a.ctl[CTL_NC + (ep + 2) % 3], 0);
+        int W = 1;
+        while (W * 2 <= a.wmax && W * 2 * ncont <= a.slots) W *= 2;
+        int team_id, member;
+        if (a.slots % (8 * W) == 0) {  // keep a team on one XCD (work-groups are dealt round-robin)
+            team_id = (b % 8) + 8 * ((b / 8) / W);
+            member = (b / 8) % W;
+        } else {
+            team_id = b / W;
+            member = b % W;
+        }
+        const int nteams = a.slots / W;
+        if (team_id < nteams && team_id < ncont) {
+            unsigned* ts = a.tsync + 2 * team_id;
+            Team team{W, member, ts, a.ctl + CTL_ERROR, 0u};
+            if (threadIdx.x == 0)  // the pair is at rest: every barrier on it ended before the grid barrier
+                team.gen = __hip_atomic_load((PRRTC_GLOBAL unsigned*)(ts + 1), __ATOMIC_ACQUIRE,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            for (int k = team_id; k < ncont; k += nteams) {
+                const unsigned long long t0 = wall_clock64();
+                const QueryView V = query_view(a, a.cont[cur * a.B + k]);
+                run_query<MAXD>(a, V, lds, team, V.qs[QS_ROUND], a.epoch_rounds > 0 ? a.epoch_rounds : -1, t0, 2,
+                                a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
+                team_barrier(team);  // members leave the query together before the next one
+            }
+        }
+        team_barrier(grid);
     }
 }
 
@@ -880,8 +929,8 @@ WsLayout ws_layout(int B, int ncap, int S) {
     w.xg = o; o = align16(o + sizeof(int) * XG_COUNT * static_cast<size_t>(S) * B);
     w.qstate = o; o = align16(o + sizeof(int) * QS_COUNT * static_cast<size_t>(B));
     w.ctl = o; o = align16(o + sizeof(int) * kCtl);
-    w.cont = o; o = align16(o + sizeof(int) * static_cast<size_t>(B));
-    w.tsync = o; o = align16(o + sizeof(unsigned) * 2 * kMaxSlots);
+    w.cont = o; o = align16(o + sizeof(int) * 3 * static_cast<size_t>(B));
+    w.tsync = o; o = align16(o + sizeof(unsigned) * 2 * (kMaxSlots + 1));
     w.stats = o; o = align16(o + 16 * sizeof(unsigned long long));
     w.total = o + 256;
     return w;
@@ -1002,8 +1051,9 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     a.slots = std::max(1, std::min(slots, kMaxSlots));
     // A lone query goes straight to a team; a batch first gives every query a short solo run.
     a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (B == 1 ? 0 : 24);
-    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 32;
-    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots);
+    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 64;
+    a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 32;
+    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
     e = small ? launch_passes<8>(a, nt, lds, stream) : launch_passes<16>(a, nt, lds, stream);

@@ -354,6 +354,7 @@ def prepare_batch(
     slots: int = 0,
     pass1_rounds: int = -1,
     team_max: int = 0,
+    epoch_rounds: int = 0,
 ) -> PreparedBatch:
     """Validates and uploads a batch; ``PreparedBatch.enqueue()`` then runs it (repeatably)."""
     collision_mode = _resolve_collision_mode(collision_checker)
@@ -445,7 +446,7 @@ def prepare_batch(
         robot, world, torch.from_numpy(np.ascontiguousarray(starts)).to(dev),
         torch.from_numpy(np.ascontiguousarray(goals)).to(dev),
         None if gcounts is None else torch.from_numpy(gcounts).to(dev), lo, hi, mnq, sd, od, attrs, bufs,
-        _lib.Launch(int(threads), int(slots), int(pass1_rounds), int(team_max)))
+        _lib.Launch(int(threads), int(slots), int(pass1_rounds), int(team_max), int(epoch_rounds)))
 
 
 def prrtc_plan_batch(
@@ -478,6 +479,7 @@ def prrtc_plan_batch(
     slots: int = 0,
     pass1_rounds: int = -1,
     team_max: int = 0,
+    epoch_rounds: int = 0,
 ) -> list[PRRTCResult]:
     """Plan a batch of independent queries in one persistent launch (prrtc.py:882-1434).
 
@@ -492,7 +494,8 @@ def prrtc_plan_batch(
         tree_ratio=tree_ratio, dynamic_domain=dynamic_domain, dd_alpha=dd_alpha, dd_radius=dd_radius,
         dd_min_radius=dd_min_radius, min_vals=min_vals, max_vals=max_vals, collision_checker=collision_checker,
         collision_margin=collision_margin, max_nodes_per_query=max_nodes_per_query, seeds=seeds, order=order,
-        path_capacity=path_capacity, threads=threads, slots=slots, pass1_rounds=pass1_rounds, team_max=team_max)
+        path_capacity=path_capacity, threads=threads, slots=slots, pass1_rounds=pass1_rounds, team_max=team_max,
+        epoch_rounds=epoch_rounds)
     pb.enqueue()
     return pb.bufs.results()
 
