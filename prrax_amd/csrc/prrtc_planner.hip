@@ -216,6 +216,15 @@ __device__ __forceinline__ unsigned long long nn_key(float dsq, int idx) {
 }
 constexpr unsigned long long kNoKey = ~0ull;
 
+// Waypoints per work unit of a check phase holding `work` waypoints: kFB (checks in order, so a
+// collision skips the rest of the unit) unless the team has more waves than units, in which case
+// units shrink (down to one waypoint) so that every wave gets one.  Uniform over the team.
+__device__ __forceinline__ int unit_size(long long work, int waves) {
+    int ku = kFB;
+    while (ku > 1 && work / ku < waves) ku >>= 1;
+    return ku;
+}
+
 // Team of W work-groups working on one query.
 struct Team {
     int W, m;
@@ -427,11 +436,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         __syncthreads();
-        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = kFB
+        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku
         // consecutive waypoints of one edge; any wave of the team claims units of any explorer
         // whose edge is not yet known to collide (FK one per lane, then checks in order).
         {
-            const int cps = (G + kFB - 1) / kFB;
+            int nact = 0;
+            for (int e0 = 0; e0 < S; e0 += nt) nact += __syncthreads_count(e0 + tid < S && state[e0 + tid] == 1);
+            const int ku = unit_size(static_cast<long long>(nact) * G, W * nwaves);
+            const int cps = (G + ku - 1) / ku;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
                 int e = -1;
@@ -454,7 +466,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                                                           __HIP_MEMORY_SCOPE_AGENT);
                 u = uniform(__shfl(u, 0, kWave));
                 if (u >= cps) continue;  // another wave took the last unit
-                const int k0 = u * kFB, nk = min(kFB, G - k0);
+                const int k0 = u * ku, nk = min(ku, G - k0);
                 if (lane < nk) {
                     const float* T = side[e] ? TB : TA;
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
@@ -582,12 +594,21 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     : 0.0f;
         }
         __syncthreads();
-        // ---- phase 5: connect walks.  Work unit = kFB consecutive waypoints of sub-edge j of one
+        // ---- phase 5: connect walks.  Work unit = ku consecutive waypoints of sub-edge j of one
         // explorer's walk; waves claim units of any explorer (claim counter per explorer), and a
         // collision at sub-edge j lowers the explorer's cap to j, so units past the first failing
         // sub-edge stop being handed out.  g_cfail[e] ends as that first failing sub-edge.
         {
-            const int cps = (G + kFB - 1) / kFB;
+            int nsub = 0;
+            for (int e0 = 0; e0 < S; e0 += nt) {
+                const int e = e0 + tid;
+                const int v = (e < S && newi[e] >= 0) ? next[e] : 0;
+                int tot;
+                block_excl_scan(v, scan, &tot);
+                nsub += tot;
+            }
+            const int ku = unit_size(static_cast<long long>(nsub) * G, W * nwaves);
+            const int cps = (G + ku - 1) / ku;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
                 int e = -1;
@@ -612,7 +633,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 if (lane == 0) u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_claim[e], 1, __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_AGENT);
                 u = uniform(__shfl(u, 0, kWave));
-                const int j = u / cps, k0 = (u - j * cps) * kFB, nk = min(kFB, G - k0);
+                const int j = u / cps, k0 = (u - j * cps) * ku, nk = min(ku, G - k0);
                 if (j >= min(next[e], uniform(gload(&V.g_cfail[e])))) continue;  // past the end or the first failure
                 if (lane < nk) {
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
