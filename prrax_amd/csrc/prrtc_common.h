@@ -94,6 +94,20 @@ __device__ __forceinline__ void global_add(unsigned long long* p, unsigned long 
     __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned long long*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Loads/stores of data shared by the work-groups of a team (COH = true): relaxed agent-scope
+// atomics, which the hardware performs coherently across the XCDs' separate L2 caches, so a
+// team barrier needs no L2 write-back/invalidate.  COH = false: plain accesses (one work-group).
+template <bool COH, class T>
+__device__ __forceinline__ T ldc(const T* p) {
+    if constexpr (COH) return __hip_atomic_load((const PRRTC_GLOBAL T*)(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool COH, class T>
+__device__ __forceinline__ void stc(T* p, T v) {
+    if constexpr (COH) __hip_atomic_store((PRRTC_GLOBAL T*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
 // ---------------------------------------------------------------------------------- scans
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -253,7 +267,7 @@ __device__ __forceinline__ bool lex_less(float da, int ia, float db, int ib) {
 // Nearest node (lowest index on ties) of kEC queries (qsh[c * kMaxDim + d], unused rows zero)
 // among tree[begin, end).  Every thread of the work-group calls it; on return red_d[c * 16] and
 // red_i[c * 16] hold query c's squared distance and index ((FLT_MAX, 0) for an empty range).
-template <int MAXD>
+template <int MAXD, bool COH = false>
 __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int end, int dim, const float* qsh,
                         float* red_d, int* red_i) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -265,7 +279,7 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
     for (int i = begin + tid; i < end; i += nt) {
         float x[MAXD];
 #pragma unroll
-        for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? tree[static_cast<size_t>(d) * ncap + i] : 0.0f;
+        for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? ldc<COH>(tree + static_cast<size_t>(d) * ncap + i) : 0.0f;
 #pragma unroll
         for (int c = 0; c < kEC; ++c) {
             float dsq = 0.0f;

@@ -233,38 +233,58 @@ struct Team {
     unsigned gen;
 };
 
-// Barrier over the team's work-groups (a plain work-group barrier when W == 1).  Thread 0 of
-// each member releases its work-group's global writes and counts in; the last arrival opens the
-// next generation.  Waiting is bounded (seconds), so a broken team raises the error flag instead
-// of hanging.
+// Waits until every global access this thread issued has completed; also a compiler barrier.
+__device__ __forceinline__ void drain_memory() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory"); }
+
+// Barrier over the team's work-groups (a plain work-group barrier when W == 1).  Everything the
+// members share goes through agent-scope atomics (ldc/stc<true>, gload/gstore), so it suffices
+// that each thread's accesses have completed before its work-group counts in: relaxed counter
+// and generation words, no L2 write-back or invalidate (a team may span XCDs).  The last
+// arrival resets the counter, waits for that store, then opens the next generation.  Waiting is
+// bounded (seconds), so a broken team raises the error flag instead of hanging.
 __device__ __forceinline__ void team_barrier(Team& t) {
-    __syncthreads();
     if (t.W > 1) {
+        drain_memory();
+        __syncthreads();
         if (threadIdx.x == 0) {
-            __threadfence();
             unsigned* cnt = t.sync;
             unsigned* gen = t.sync + 1;
-            const unsigned prev = __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)cnt, 1u, __ATOMIC_ACQ_REL,
+            const unsigned prev = __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)cnt, 1u, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT);
             if (prev == static_cast<unsigned>(t.W) - 1u) {
                 __hip_atomic_store((PRRTC_GLOBAL unsigned*)cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                drain_memory();
+                __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 unsigned spins = 0;
-                while (__hip_atomic_load((PRRTC_GLOBAL unsigned*)gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                while (__hip_atomic_load((PRRTC_GLOBAL unsigned*)gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
                        t.gen) {
-                    __builtin_amdgcn_s_sleep(2);
-                    if (++spins > (1u << 25)) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1u << 26)) {
                         gstore(t.error, 1);
                         break;
                     }
                 }
             }
-            __threadfence();
+            drain_memory();
         }
         t.gen += 1u;
         __syncthreads();
+    } else {
+        __syncthreads();
     }
+}
+
+// Barrier over the whole grid between epochs: as team_barrier, plus agent-scope release/acquire
+// fences (L2 write-back and invalidate), so plain stores of one epoch (continue lists, saved
+// round state) are visible to the work-groups of the next.
+__device__ __forceinline__ void grid_barrier(Team& t) {
+    drain_memory();  // the write-back below must see every wave's stores
+    __syncthreads();
+    if (threadIdx.x == 0) __threadfence();
+    team_barrier(t);
+    if (threadIdx.x == 0) __threadfence();
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------- one query
@@ -279,7 +299,7 @@ struct QueryView {
     int* qs;
 };
 
-template <int MAXD>
+template <int MAXD, bool COH>
 __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team& team, int r_begin, int r_budget,
                           unsigned long long t0, int pass, int* push_cnt, int* push_list) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -407,7 +427,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                         ? sample_dim(a.min_vals, a.max_vals, primes + 16 * list[c0 + c], d, r) : 0.0f;
                 }
                 __syncthreads();
-                nn_tile<MAXD>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
+                nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
                 if (tid < nc) gmin64(&V.g_key[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
                 __syncthreads();
             }
@@ -423,13 +443,13 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const int ni = static_cast<int>(key & 0xffffffffull);
             nn[e] = ni;
             const float nd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
-            if (a.dd && R[ni] < nd) {
+            if (a.dd && ldc<COH>(R + ni) < nd) {
                 state[e] = 0;
             } else {
                 const float s = (nd > 1e-8f) ? fminf(1.0f, a.step_size / nd) : 1.0f;
                 for (int d = 0; d < dim; ++d) {
                     const float smp = sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r);
-                    const float base = T[static_cast<size_t>(d) * ncap + ni];
+                    const float base = ldc<COH>(T + static_cast<size_t>(d) * ncap + ni);
                     cand[e * dim + d] = base + (smp - base) * s;
                 }
                 state[e] = 1;
@@ -471,7 +491,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     const float* T = side[e] ? TB : TA;
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
                     for (int d = 0; d < dim; ++d) {
-                        const float base = T[static_cast<size_t>(d) * ncap + nn[e]];
+                        const float base = ldc<COH>(T + static_cast<size_t>(d) * ncap + nn[e]);
                         L.q[d * kFBP] = base + tf * (cand[e * dim + d] - base);
                     }
                     if (M.enabled) fk_lane(M, L);
@@ -514,9 +534,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                             ni = idx;
                             if (e % W == m) {
                                 float* T = t ? TB : TA;
-                                for (int d = 0; d < dim; ++d) T[static_cast<size_t>(d) * ncap + idx] = cand[e * dim + d];
-                                (t ? PB : PA)[idx] = nn[e];
-                                (t ? RB : RA)[idx] = FLT_MAX;
+                                for (int d = 0; d < dim; ++d)
+                                    stc<COH>(T + static_cast<size_t>(d) * ncap + idx, cand[e * dim + d]);
+                                stc<COH>((t ? PB : PA) + idx, nn[e]);
+                                stc<COH>((t ? RB : RA) + idx, FLT_MAX);
                             }
                         } else {
                             sc[SC_OVERFLOW] = 1;
@@ -537,10 +558,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     const int key = hk[h];
                     if (key < 0) continue;
                     float* R = (key & 1) ? RB : RA;
-                    float rad = R[key >> 1];
+                    float rad = ldc<COH>(R + (key >> 1));
                     for (int i = 0; i < hs[h]; ++i) radius_shrink(rad, a.dd_alpha, a.dd_radius, a.dd_min_radius);
                     for (int i = 0; i < hg[h]; ++i) radius_grow(rad, a.dd_alpha);
-                    R[key >> 1] = rad;
+                    stc<COH>(R + (key >> 1), rad);
                     hk[h] = -1; hs[h] = 0; hg[h] = 0;
                 }
             }
@@ -570,7 +591,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                         qsh[c * kMaxDim + d] = (c < nc && d < dim) ? cand[list[c0 + c] * dim + d] : 0.0f;
                     }
                     __syncthreads();
-                    nn_tile<MAXD>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
+                    nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
                     if (tid < nc) gmin64(&V.g_okey[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
                     __syncthreads();
                 }
@@ -590,7 +611,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             next[e] = n_ext;
             for (int d = 0; d < dim; ++d)
                 vec[e * dim + d] = (n_ext > 0)
-                    ? (O[static_cast<size_t>(d) * ncap + oi] - cand[e * dim + d]) / static_cast<float>(n_ext)
+                    ? (ldc<COH>(O + static_cast<size_t>(d) * ncap + oi) - cand[e * dim + d]) / static_cast<float>(n_ext)
                     : 0.0f;
         }
         __syncthreads();
@@ -690,10 +711,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                                 float c = cand[e * dim + d];
                                 const float v = vec[e * dim + d];
                                 for (int jj = 0; jj <= k; ++jj) c += v;
-                                T[static_cast<size_t>(d) * ncap + idx] = c;
+                                stc<COH>(T + static_cast<size_t>(d) * ncap + idx, c);
                             }
-                            P[idx] = parent;
-                            R[idx] = FLT_MAX;
+                            stc<COH>(P + idx, parent);
+                            stc<COH>(R + idx, FLT_MAX);
                             parent = idx;
                         }
                     }
@@ -778,7 +799,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 int na = 0, cur = sc[SC_CONN0];
                 for (int it = 0; it < ncap; ++it) {
                     ++na;
-                    const int p = PA[cur];
+                    const int p = ldc<COH>(PA + cur);
                     if (p == cur) break;
                     cur = p;
                 }
@@ -786,7 +807,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 cur = sc[SC_CONN1];
                 for (int it = 0; it < ncap; ++it) {
                     ++nb;
-                    const int p = PB[cur];
+                    const int p = ldc<COH>(PB + cur);
                     if (p == cur) break;
                     cur = p;
                 }
@@ -795,13 +816,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 } else {
                     cur = sc[SC_CONN0];
                     for (int k = na - 1; k >= 0; --k) {
-                        for (int d = 0; d < dim; ++d) out[k * dim + d] = TA[static_cast<size_t>(d) * ncap + cur];
-                        cur = PA[cur];
+                        for (int d = 0; d < dim; ++d) out[k * dim + d] = ldc<COH>(TA + static_cast<size_t>(d) * ncap + cur);
+                        cur = ldc<COH>(PA + cur);
                     }
                     cur = sc[SC_CONN1];
                     for (int k = 0; k < nb; ++k) {
-                        for (int d = 0; d < dim; ++d) out[(na + k) * dim + d] = TB[static_cast<size_t>(d) * ncap + cur];
-                        cur = PB[cur];
+                        for (int d = 0; d < dim; ++d)
+                            out[(na + k) * dim + d] = ldc<COH>(TB + static_cast<size_t>(d) * ncap + cur);
+                        cur = ldc<COH>(PB + cur);
                     }
                     len = na + nb;
                 }
@@ -878,7 +900,7 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
         const unsigned long long t0 = wall_clock64();
         const QueryView V = query_view(a, a.order ? a.order[qi] : qi);
         init_query(a, V);
-        run_query<MAXD>(a, V, lds, team, 1, a.pass1_rounds, t0, 1, a.ctl + CTL_NC, a.cont);
+        run_query<MAXD, false>(a, V, lds, team, 1, a.pass1_rounds, t0, 1, a.ctl + CTL_NC, a.cont);
     }
 }
 
@@ -920,12 +942,12 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
             for (int k = team_id; k < ncont; k += nteams) {
                 const unsigned long long t0 = wall_clock64();
                 const QueryView V = query_view(a, a.cont[cur * a.B + k]);
-                run_query<MAXD>(a, V, lds, team, V.qs[QS_ROUND], a.epoch_rounds > 0 ? a.epoch_rounds : -1, t0, 2,
+                run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], a.epoch_rounds > 0 ? a.epoch_rounds : -1, t0, 2,
                                 a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
                 team_barrier(team);  // members leave the query together before the next one
             }
         }
-        team_barrier(grid);
+        grid_barrier(grid);
     }
 }
 
