@@ -367,6 +367,42 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         ph[k] += now - tph;
         tph = now;
     };
+    // Nearest neighbours of the explorers that tree(e) picks (0 / 1 / -1 = none), query point
+    // qpt(e, d), into keys[e].  Work: tiles of kEC explorers (tree-0 tiles, then tree-1 tiles).
+    // With W >= tiles, tile k goes to the W / tiles members m = k (mod tiles), each scanning its
+    // part of the tree; otherwise members take whole tiles in turn.  Partial results merge
+    // through a 64-bit minimum of (squared distance, index), which equals the serial scan's.
+    auto nn_phase = [&](auto tree, auto qpt, unsigned long long* keys, int sz0, int sz1) {
+        const int n0 = compact(S, [&](int e) { return tree(e) == 0; }, list, scan);
+        const int n1 = compact(S, [&](int e) { return tree(e) == 1; }, list + n0, scan);
+        const int nt0 = (n0 + kEC - 1) / kEC, ntl = nt0 + (n1 + kEC - 1) / kEC;
+        if (ntl == 0) return;
+        int first = m, stride = W, parts = 1, part = 0;
+        if (W >= ntl) {
+            first = m % ntl;
+            stride = ntl;
+            parts = W / ntl;
+            part = m / ntl;
+            if (part >= parts) return;
+        }
+        for (int k = first; k < ntl; k += stride) {
+            const int t = (k < nt0) ? 0 : 1;
+            const int c0 = t ? n0 + (k - nt0) * kEC : k * kEC;
+            const int nc = min(kEC, (t ? n0 + n1 : n0) - c0);
+            const int sz = t ? sz1 : sz0;
+            const int lo = static_cast<int>((static_cast<long long>(sz) * part) / parts);
+            const int hi = static_cast<int>((static_cast<long long>(sz) * (part + 1)) / parts);
+            if (lo >= hi) continue;
+            for (int q = tid; q < kEC * MAXD; q += nt) {
+                const int c = q / MAXD, d = q - (q / MAXD) * MAXD;
+                qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qpt(list[c0 + c], d) : 0.0f;
+            }
+            __syncthreads();
+            nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
+            if (tid < nc) gmin64(&keys[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
+            __syncthreads();
+        }
+    };
     const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
     int r = r_begin;
     for (;; ++r) {
@@ -411,27 +447,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         __syncthreads();
-        // ---- phase 1b: nearest neighbour of each explorer's sample in its tree; tiles of kEC
-        // explorers, each member scans its slice of the tree and merges through a key minimum
-        for (int t = 0; t < 2; ++t) {
-            const int nl = compact(S, [&](int e) { return side[e] == t; }, list, scan);
-            const int sz = t ? size_b : size_a;
-            const int lo = static_cast<int>((static_cast<long long>(sz) * m) / W);
-            const int hi = static_cast<int>((static_cast<long long>(sz) * (m + 1)) / W);
-            if (lo >= hi) continue;
-            for (int c0 = 0; c0 < nl; c0 += kEC) {
-                const int nc = min(kEC, nl - c0);
-                for (int k = tid; k < kEC * MAXD; k += nt) {
-                    const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
-                    qsh[c * kMaxDim + d] = (c < nc && d < dim)
-                        ? sample_dim(a.min_vals, a.max_vals, primes + 16 * list[c0 + c], d, r) : 0.0f;
-                }
-                __syncthreads();
-                nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
-                if (tid < nc) gmin64(&V.g_key[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
-                __syncthreads();
-            }
-        }
+        // ---- phase 1b: nearest neighbour of each explorer's sample in its tree
+        nn_phase([&](int e) { return side[e]; },
+                 [&](int e, int d) { return sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r); }, V.g_key,
+                 size_a, size_b);
         team_barrier(team);  // B1: nearest neighbours of all explorers are in global memory
         mark(0);
         // ---- phase 1c: dynamic-domain skip and steer, every explorer (redundant per member)
@@ -575,28 +594,9 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         team_barrier(team);  // B4: new nodes and radii are in global memory
         mark(2);
         if (sc[SC_STATUS] != 0) break;
-        // ---- phase 4: connect nearest neighbour in the other tree
-        {
-            const int sa = sc[SC_SIZE_A], sb = sc[SC_SIZE_B];
-            for (int t = 0; t < 2; ++t) {  // t: tree searched (the other tree of the extended one)
-                const int nl = compact(S, [&](int e) { return newi[e] >= 0 && side[e] == 1 - t; }, list, scan);
-                const int sz = t ? sb : sa;
-                const int lo = static_cast<int>((static_cast<long long>(sz) * m) / W);
-                const int hi = static_cast<int>((static_cast<long long>(sz) * (m + 1)) / W);
-                if (lo >= hi) continue;
-                for (int c0 = 0; c0 < nl; c0 += kEC) {
-                    const int nc = min(kEC, nl - c0);
-                    for (int k = tid; k < kEC * MAXD; k += nt) {
-                        const int c = k / MAXD, d = k - (k / MAXD) * MAXD;
-                        qsh[c * kMaxDim + d] = (c < nc && d < dim) ? cand[list[c0 + c] * dim + d] : 0.0f;
-                    }
-                    __syncthreads();
-                    nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
-                    if (tid < nc) gmin64(&V.g_okey[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
-                    __syncthreads();
-                }
-            }
-        }
+        // ---- phase 4: connect nearest neighbour in the other tree of each new node
+        nn_phase([&](int e) { return newi[e] >= 0 ? 1 - side[e] : -1; },
+                 [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, sc[SC_SIZE_A], sc[SC_SIZE_B]);
         team_barrier(team);  // B5: connect nearest neighbours are in global memory
         mark(3);
         for (int e = tid; e < S; e += nt) {
