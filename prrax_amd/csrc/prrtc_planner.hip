@@ -1094,7 +1094,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     a.slots = std::max(1, std::min(slots, kMaxSlots));
     // A lone query goes straight to a team; a batch first gives every query a short solo run.
     a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (B == 1 ? 0 : 24);
-    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 64;
+    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 256;
     a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 32;
     hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1);
     hipError_t e = hipGetLastError();
