@@ -25,6 +25,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (datasheet)
 PEAK_HBM_GBS = 8000.0
+# HBM bytes of one batched step of the default 1024-query workload (plan_pass1 + plan_pass2),
+# from rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) and WRITE_SIZE (x 1.0).
+TRAFFIC_BYTES_PER_STEP = 24.99e9
 
 METRIC = "plans/sec + p50 ms/plan, batched vs per-query at equal solve rate; FK+CC cfg/s"
 
@@ -210,6 +213,8 @@ def main():
                       "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
         log(f"fkcc {mode}: {fkcc[mode]}")
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
+    default_workload = (args.queries == 1024 and args.samples == 128 and args.granularity == 16
+                        and args.max_iterations == 10_000)
     out = {
         "metric": METRIC,
         "value": round(plans_per_s, 2),
@@ -235,9 +240,9 @@ def main():
         "batched_over_per_query": round(plans_per_s / pq["plans_per_s"], 2) if pq else None,
         "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
                              "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
-        "roofline": {"kernel": "plan_kernel", "bound": "valu", "achieved": round(achieved, 4),
+        "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-                     "traffic": None,
+                     "traffic": TRAFFIC_BYTES_PER_STEP if default_workload else None,
                      "flops_per_launch": stats["flops"]},
         "device": dev_name,
     }

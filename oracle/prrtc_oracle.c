@@ -39,6 +39,20 @@
  * Floating-point contract (shared with the HIP library, written twice, independently):
  * compiled with -ffp-contract=off; every fused multiply-add is an explicit fmaf(); sin/cos
  * come from the polynomial below (not libm), so CPU and GPU round identically.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   oc_halton, oc_prime_order ...... known answers, closed-form radical inverse, canonical order
+ *   oc_sincos ...................... libm accuracy over +-4 pi, symmetries
+ *   oc_fk .......................... FP64 SE3 FK (1e-5), q = 0, single revolute = Rodrigues,
+ *                                    mimic = driving joint, a transposed rotation is caught
+ *   distances / predicates ......... FP64 closed forms and FP64 predicates away from contact
+ *   oc_nearest ..................... brute-force argmin with injected exact ties
+ *   oc_plan ........................ properties (collision-free nodes/paths under FP64, forest
+ *                                    rooted at the roots, dynamic-domain rules, caps, Halton
+ *                                    first round, determinism)
+ * Parity unpinned: pyroffi's own FP32 operation order in fk_single / apply_se3_point /
+ * sphere_*_dist / colldist_from_sdf (the reference's dependency is absent, so its exact FP32
+ * results cannot be reproduced; these functions are pinned to the mathematics instead).
  */
 #include <float.h>
 #include <math.h>

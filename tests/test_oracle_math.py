@@ -82,3 +82,36 @@ def test_sincos_symmetries():
         assert abs(s * s + c * c - 1.0) < 5e-7
         s2, c2 = O.sincos(float(-x))
         assert s2 == -s and c2 == c
+
+
+@pytest.mark.parametrize("n,dim", [(1, 7), (37, 7), (1000, 7), (513, 14)])
+def test_nearest_matches_brute_force_with_ties(n, dim):
+    """oc_nearest is the lowest-index argmin of the squared distance (prrtc_robogpu.py:82-87
+    semantics): brute force in float64, exact duplicates injected so ties must go low."""
+    rng = np.random.default_rng(n * 31 + dim)
+    max_nodes = n + 5
+    pts = rng.uniform(-3, 3, size=(n, dim)).astype(np.float32)
+    if n > 10:
+        pts[n // 2] = pts[3]          # duplicate node: index 3 must win
+        pts[n - 1] = pts[n // 3]      # and another pair
+    tree = np.zeros((dim, max_nodes), np.float32)
+    tree[:, :n] = pts.T
+    tree[:, n:] = -7.0                # beyond size: must be ignored
+    qs = rng.uniform(-3, 3, size=(64, dim)).astype(np.float32)
+    if n > 10:
+        qs[0] = pts[n // 2]           # exact hit on a duplicated node
+        qs[1] = pts[n - 1]
+    idx, dsq = O.nearest(tree, n, qs)
+    d64 = ((qs[:, None, :].astype(np.float64) - pts[None].astype(np.float64)) ** 2).sum(-1)
+    want = d64.argmin(axis=1)         # numpy: first (lowest) index among equal minima
+    for k in range(qs.shape[0]):
+        srt = np.sort(d64[k])
+        gap = srt[1] - srt[0] if srt.size > 1 else np.inf
+        if gap > 1e-5 * max(srt[0], 1e-12):   # clear winner: FP32 must find it
+            assert idx[k] == want[k], k
+        else:                                  # FP32 may pick either of two near-equal nodes
+            assert d64[k, idx[k]] - srt[0] <= 1e-5 * max(srt[0], 1e-12), k
+        assert abs(dsq[k] - d64[k, idx[k]]) <= 1e-5 * max(d64[k, idx[k]], 1.0)
+    if n > 10:
+        assert idx[0] == 3 and dsq[0] == 0.0
+        assert idx[1] == n // 3 and dsq[1] == 0.0
