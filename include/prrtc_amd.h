@@ -181,6 +181,32 @@ int prrtc_extend(prrtc_stream_t stream, const float* tree_configs, int32_t dim, 
                  const int32_t* nearest, const float* samples, int32_t batch, float step_size, float* out_configs,
                  int32_t* out_parents, int32_t* out_valid);
 
+/* Differentiable sphere-penetration cost (no reference counterpart; the nearest reference code
+ * is the margin predicate, prrtc_planner.cu:314-387).  For each of n configs [n, dim]:
+ *   cost = sum over robot spheres i, world spheres w of softplus_beta(r_i + r_w - |c_i - o_w|),
+ *   softplus_beta(x) = log(1 + exp(beta x)) / beta,
+ * and, when out_grad [n, dim] is not NULL, d cost / d config through the FK chain (hand-written
+ * adjoint; FK is recomputed, no per-link transforms are stored).  world_spheres [n_ws, 4]. */
+int prrtc_sphere_cost(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim, const prrtc_robot* robot,
+                      const float* world_spheres, int32_t n_world_spheres, float beta, float* out_cost,
+                      float* out_grad, uint64_t* work_flops);
+
+/* Workspace bytes of prrtc_refine_paths. */
+size_t prrtc_refine_workspace_bytes(int32_t n_paths, int32_t n_waypoints, int32_t dim);
+
+/* Path refinement (no reference counterpart): `iterations` Jacobi gradient steps on the
+ * interior waypoints of paths [n_paths, n_waypoints, dim] (updated in place) of
+ *   sum of prrtc_sphere_cost over all waypoints (world->world_spheres) + smooth_weight * sum |q_{k+1} - q_k|^2,
+ * each candidate clamped to [min_vals, max_vals] (both NULL = no clamp) and kept only where the
+ * exact predicate (collision_mode/margin against the whole world, as prrtc_check_configs) says
+ * free; end points never move.  out_cost [iterations + 1] (may be NULL): the total cost before
+ * each step and after the last. */
+int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int32_t n_waypoints, int32_t dim,
+                       const float* min_vals, const float* max_vals, const prrtc_robot* robot,
+                       const prrtc_world* world, int32_t collision_mode, float collision_margin, float beta,
+                       float smooth_weight, float step, int32_t iterations, float* out_cost, void* workspace,
+                       size_t workspace_bytes, uint64_t* work_flops);
+
 /* Message of the last failing call on this thread. */
 const char* prrtc_last_error(void);
 

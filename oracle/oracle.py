@@ -281,6 +281,11 @@ def quat_mat64(q) -> np.ndarray:
 
 def fk64(cc: dict, q) -> np.ndarray:
     """FP64 FK: ``[n_joints, 4, 4]`` world transforms in original joint order."""
+    return _fk64_sorted(cc, q)[np.asarray(cc["fk_topo_inv"])]
+
+
+def _fk64_sorted(cc: dict, q) -> np.ndarray:
+    """FP64 FK in the context's (topological) joint order."""
     q = np.asarray(q, np.float64)
     tw = np.asarray(cc["fk_twists"], np.float64)
     ptf = np.asarray(cc["fk_parent_tf"], np.float64)
@@ -289,7 +294,6 @@ def fk64(cc: dict, q) -> np.ndarray:
     mim = np.asarray(cc["fk_mimic_act_idx"])
     mul = np.asarray(cc["fk_mimic_mul"], np.float64)
     off = np.asarray(cc["fk_mimic_off"], np.float64)
-    inv = np.asarray(cc["fk_topo_inv"])
     n = par.shape[0]
     T = np.zeros((n, 4, 4))
     for j in range(n):
@@ -304,7 +308,7 @@ def fk64(cc: dict, q) -> np.ndarray:
         P[:3, 3] = ptf[j, 4:]
         base = T[par[j]] if par[j] >= 0 else np.eye(4)
         T[j] = base @ P @ se3_exp64(tw[j], th)
-    return T[inv]
+    return T
 
 
 def sphere_positions64(cc: dict, q) -> np.ndarray:
@@ -361,3 +365,71 @@ def pair_distances64(cc: dict, q) -> np.ndarray:
 def min_distance64(cc: dict, q) -> float:
     d = pair_distances64(cc, q)
     return float(d.min()) if d.size else float("inf")
+
+
+# ---------------------------------------------------------------- sphere-penetration cost (FP64)
+def _softplus64(z):
+    return np.maximum(z, 0.0) + np.log1p(np.exp(-np.abs(z)))
+
+
+def sphere_cost64(cc: dict, q, beta: float = 50.0) -> float:
+    """sum over robot spheres i, world spheres w of softplus_beta(r_i + r_w - |c_i - o_w|), FP64."""
+    P = sphere_positions64(cc, q)
+    ok = ~np.isnan(P[:, 0])
+    r = np.asarray(cc["sphere_radius"], np.float64)[ok]
+    ws = np.asarray(cc.get("world_spheres", np.zeros((0, 4))), np.float64).reshape(-1, 4)
+    if ws.shape[0] == 0 or not ok.any():
+        return 0.0
+    d = np.linalg.norm(P[ok][:, None, :] - ws[None, :, :3], axis=-1)
+    x = r[:, None] + ws[None, :, 3] - d
+    return float(_softplus64(beta * x).sum() / beta)
+
+
+def sphere_cost_grad64(cc: dict, q, beta: float = 50.0) -> np.ndarray:
+    """Analytic gradient of sphere_cost64 through the kinematic tree (adjoint form), FP64:
+    sphere forces g_i, subtree sums F_j / M_j, then w_j . (M_j - t_j x F_j) + (R_j v_j) . F_j."""
+    q = np.asarray(q, np.float64)
+    Ts = _fk64_sorted(cc, q)
+    n = Ts.shape[0]
+    inv = np.asarray(cc["fk_topo_inv"])
+    par = np.asarray(cc["fk_parent_idx"])
+    links = np.asarray(cc["sphere_link_idx"])
+    loc = np.asarray(cc["sphere_local"], np.float64)
+    rad = np.asarray(cc["sphere_radius"], np.float64)
+    ws = np.asarray(cc.get("world_spheres", np.zeros((0, 4))), np.float64).reshape(-1, 4)
+    F = np.zeros((n, 3))
+    M = np.zeros((n, 3))
+    for s, l in enumerate(links):
+        if not (0 <= l < n):
+            continue
+        j = inv[l]
+        c = Ts[j, :3, :3] @ loc[s] + Ts[j, :3, 3]
+        diff = c[None, :] - ws[:, :3]
+        d = np.linalg.norm(diff, axis=1)
+        z = beta * (rad[s] + ws[:, 3] - d)
+        sig = 0.5 * (1.0 + np.tanh(0.5 * z))
+        good = d > 0
+        g = -(sig[good, None] * diff[good] / d[good, None]).sum(axis=0)
+        F[j] += g
+        M[j] += np.cross(c, g)
+    for j in range(n - 1, -1, -1):
+        if par[j] >= 0:
+            F[par[j]] += F[j]
+            M[par[j]] += M[j]
+    tw = np.asarray(cc["fk_twists"], np.float64)
+    gj = np.zeros(n)
+    for j in range(n):
+        R, t = Ts[j, :3, :3], Ts[j, :3, 3]
+        w = R @ tw[j, 3:]
+        v = R @ tw[j, :3]
+        gj[j] = w @ (M[j] - np.cross(t, F[j])) + v @ F[j]
+    act = np.asarray(cc["fk_act_idx"])
+    mim = np.asarray(cc["fk_mimic_act_idx"])
+    mul = np.asarray(cc["fk_mimic_mul"], np.float64)
+    out = np.zeros(q.shape[0])
+    for j in range(n):
+        if mim[j] >= 0:
+            out[mim[j]] += mul[j] * gj[j]
+        elif act[j] >= 0:
+            out[act[j]] += gj[j]
+    return out

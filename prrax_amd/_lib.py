@@ -37,6 +37,9 @@ EXPORTED_SYMBOLS = (
     "prrtc_forward_kinematics",
     "prrtc_nearest_neighbor",
     "prrtc_extend",
+    "prrtc_sphere_cost",
+    "prrtc_refine_workspace_bytes",
+    "prrtc_refine_paths",
     "prrtc_last_error",
     "prrtc_version",
 )
@@ -123,6 +126,14 @@ def lib() -> ctypes.CDLL:
     L.prrtc_nearest_neighbor.restype = ctypes.c_int
     L.prrtc_extend.argtypes = [_vp, _vp, i32, i32, _vp, _vp, i32, ctypes.c_float, _vp, _vp, _vp]
     L.prrtc_extend.restype = ctypes.c_int
+    f32 = ctypes.c_float
+    L.prrtc_sphere_cost.argtypes = [_vp, _vp, i32, i32, P(Robot), _vp, i32, f32, _vp, _vp, _vp]
+    L.prrtc_sphere_cost.restype = ctypes.c_int
+    L.prrtc_refine_workspace_bytes.argtypes = [i32, i32, i32]
+    L.prrtc_refine_workspace_bytes.restype = sz
+    L.prrtc_refine_paths.argtypes = [_vp, _vp, i32, i32, i32, _vp, _vp, P(Robot), P(World), i32, f32, f32, f32, f32,
+                                     i32, _vp, _vp, sz, _vp]
+    L.prrtc_refine_paths.restype = ctypes.c_int
     L.prrtc_last_error.argtypes = []
     L.prrtc_last_error.restype = ctypes.c_char_p
     L.prrtc_version.argtypes = []

@@ -1,0 +1,427 @@
+// prrtc_refine.hip -- differentiable sphere-penetration cost through FK, and path refinement.
+//
+// No reference counterpart (the reference has no cost gradient; its nearest relatives are the
+// margin predicate, prrtc_planner.cu:314-387, and pyroffi's absent collision-cost kernel called
+// from benchmark_collision_checkers.py:11-15).  Cost of one configuration q:
+//
+//   C(q) = sum over robot spheres i, world spheres w of softplus_b(r_i + r_w - |c_i(q) - o_w|),
+//   softplus_b(x) = log(1 + exp(b x)) / b,
+//
+// with c_i(q) the sphere centre through FK.  The adjoint goes through the kinematic tree
+// without storing per-link transforms: FK is recomputed in the gradient kernel, and
+//   dC/dc_i = -sum_w sigmoid(b x_iw) (c_i - o_w) / |c_i - o_w|            (force g_i)
+//   F_j = sum of g_i over spheres on joint j's subtree, M_j = sum of c_i x g_i over the same,
+//   dC/dtheta_j = w_j . (M_j - t_j x F_j) + (R_j v_j) . F_j,
+// where (R_j, t_j) is joint j's world frame and (v_j, omega_j) its twist (w_j = R_j omega_j):
+// moving theta_j moves every point x of the subtree by w_j x (x - t_j) + R_j v_j.  Joint
+// gradients then flow to the configuration through the act / mimic maps of the FK.
+//
+// Layout: one wave per configuration, lanes over robot spheres, world spheres broadcast from
+// LDS; per-sphere (joint, g, c x g) go to a per-wave LDS area and are summed per joint in sphere
+// order by one lane per joint, so results are deterministic.  FP32 throughout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "prrtc_amd.h"
+#include "prrtc_common.h"
+#include "prrtc_host.h"
+
+namespace prrtc {
+
+// Algorithmic flops (FMA = 2) per robot-world sphere pair: cost 17 (difference 3, squared
+// norm 5, sqrt, penetration 2, scale, exp, log1p, add, divide), gradient 11 more (sigmoid 3,
+// 1/d and scale 2, accumulate 6); per joint of the adjoint 30.
+constexpr unsigned kFlopsPairCost = 17, kFlopsPairGrad = 11, kFlopsJointAdj = 30;
+
+struct CostArgs {
+    RobotDev robot;
+    WorldDev world;
+    ModelLayout ml;
+    size_t off_scan, off_acc;
+    int acc_stride;  // floats per wave in the accumulator area
+    const float* configs;
+    int n, dim;
+    float beta;
+    float* out_cost;
+    float* out_grad;  // [n, dim] or null
+    unsigned long long* flops;
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <bool GRAD>
+__global__ __launch_bounds__(256) void sphere_cost_kernel(CostArgs a) {
+    extern __shared__ __align__(16) char lds[];
+    int* scan = reinterpret_cast<int*>(lds + a.off_scan);
+    Model M;
+    stage_model(M, lds, a.ml, a.robot, a.world, 0, a.dim, 0, 0.0f, scan);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const WaveArea W = wave_area(lds, a.ml, M.n_joints);
+    LaneScratch L;
+    L.q = W.q + lane;
+    L.T = W.T + lane;
+    L.nt = kFBP;
+    const int nj = M.n_joints, nrs = M.n_spheres;
+    // accumulator area of this wave: [nrs][7] sphere terms, [nj][6] joint sums, [nj] joint grads
+    float* acc = reinterpret_cast<float*>(lds + a.off_acc) + static_cast<size_t>(wid) * a.acc_stride;
+    float* jsum = acc + 7 * nrs;
+    float* jgrad = jsum + 6 * nj;
+    const float beta = a.beta, inv_beta = 1.0f / a.beta;
+    const int wpb = blockDim.x / kWave;
+    const int total_waves = gridDim.x * wpb;
+    unsigned long long flops = 0;
+    const int nb = (a.n + kFB - 1) / kFB;
+    for (int b = blockIdx.x * wpb + wid; b < nb; b += total_waves) {
+        const int i = b * kFB + lane;
+        if (lane < kFB && i < a.n) {
+            for (int d = 0; d < a.dim; ++d) L.q[d * kFBP] = a.configs[static_cast<size_t>(i) * a.dim + d];
+            if (nj > 0) fk_lane(M, L);
+        }
+        wave_sync();
+        const int nc = min(kFB, a.n - b * kFB);
+        for (int c = 0; c < nc; ++c) {
+            const int idx = b * kFB + c;
+            const float* T = W.T + c;
+            float cost = 0.0f;
+            int active = 0;
+            for (int s = lane; s < nrs; s += kWave) {
+                const int js = M.sphere_js[s];
+                float gx = 0.0f, gy = 0.0f, gz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+                if (js >= 0) {
+                    ++active;
+                    Mat3 R;
+                    float tx, ty, tz;
+                    frame_from(T, kFBP, js, R, tx, ty, tz);
+                    const float4 sp = M.spheres[s];
+                    const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+                    cx = p.x; cy = p.y; cz = p.z;
+                    for (int w = 0; w < M.n_ws; ++w) {
+                        const float4 o = M.wsph[w];
+                        const float dx = cx - o.x, dy = cy - o.y, dz = cz - o.z;
+                        const float d = __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+                        const float z = beta * ((sp.w + o.w) - d);
+                        // softplus(z) = max(z, 0) + log1p(exp(-|z|)), stable for either sign
+                        cost += (fmaxf(z, 0.0f) + log1pf(expf(-fabsf(z)))) * inv_beta;
+                        if (GRAD && d > 0.0f) {
+                            const float sg = 1.0f / (1.0f + expf(-z));  // d softplus_b / dx
+                            const float k = -sg / d;
+                            gx = __builtin_fmaf(k, dx, gx);
+                            gy = __builtin_fmaf(k, dy, gy);
+                            gz = __builtin_fmaf(k, dz, gz);
+                        }
+                    }
+                }
+                if (GRAD) {
+                    float* e = acc + 7 * s;
+                    e[0] = __int_as_float(js);
+                    e[1] = gx; e[2] = gy; e[3] = gz;
+                    e[4] = __builtin_fmaf(cy, gz, -(cz * gy));
+                    e[5] = __builtin_fmaf(cz, gx, -(cx * gz));
+                    e[6] = __builtin_fmaf(cx, gy, -(cy * gx));
+                }
+            }
+            cost = wave_sum(cost);
+            if (lane == 0) a.out_cost[idx] = cost;
+            // flops of this config: FK, sphere transforms, pair terms (+ adjoint)
+            const int act_total = wave_sum_int(active);
+            if (lane == 0) {
+                flops += static_cast<unsigned long long>(kFlopsJoint) * nj +
+                         static_cast<unsigned long long>(act_total) *
+                             (kFlopsXf + static_cast<unsigned long long>(M.n_ws) *
+                                             (kFlopsPairCost + (GRAD ? kFlopsPairGrad : 0u)));
+                if (GRAD) flops += static_cast<unsigned long long>(kFlopsJointAdj) * nj;
+            }
+            if (GRAD) {
+                wave_sync();
+                // per-joint force and moment sums, in sphere order
+                for (int j = lane; j < nj; j += kWave) {
+                    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
+                    for (int s = 0; s < nrs; ++s) {
+                        const float* e = acc + 7 * s;
+                        if (__float_as_int(e[0]) != j) continue;
+                        f0 += e[1]; f1 += e[2]; f2 += e[3];
+                        m0 += e[4]; m1 += e[5]; m2 += e[6];
+                    }
+                    float* o = jsum + 6 * j;
+                    o[0] = f0; o[1] = f1; o[2] = f2; o[3] = m0; o[4] = m1; o[5] = m2;
+                }
+                wave_sync();
+                // subtree sums: children follow their parents in joint order
+                if (lane == 0) {
+                    for (int j = nj - 1; j > 0; --j) {
+                        const int p = M.joints[j].parent;
+                        if (p < 0) continue;
+                        for (int k = 0; k < 6; ++k) jsum[6 * p + k] += jsum[6 * j + k];
+                    }
+                }
+                wave_sync();
+                for (int j = lane; j < nj; j += kWave) {
+                    const JointRec& J = M.joints[j];
+                    float g = 0.0f;
+                    if (J.type != kFixed) {
+                        Mat3 R;
+                        float tx, ty, tz;
+                        frame_from(T, kFBP, j, R, tx, ty, tz);
+                        const float* f = jsum + 6 * j;
+                        if (J.type == kRot || J.type == kScrew) {
+                            // world axis times |omega|; moment about the joint origin
+                            const float3 w = mat_apply(R, 0.0f, 0.0f, 0.0f, J.wn * J.ax, J.wn * J.ay, J.wn * J.az);
+                            const float mx = f[3] - __builtin_fmaf(ty, f[2], -(tz * f[1]));
+                            const float my = f[4] - __builtin_fmaf(tz, f[0], -(tx * f[2]));
+                            const float mz = f[5] - __builtin_fmaf(tx, f[1], -(ty * f[0]));
+                            g = __builtin_fmaf(w.z, mz, __builtin_fmaf(w.y, my, w.x * mx));
+                        }
+                        if (J.type == kPrism || J.type == kScrew) {
+                            const float3 v = mat_apply(R, 0.0f, 0.0f, 0.0f, J.vx, J.vy, J.vz);
+                            g = __builtin_fmaf(v.z, f[2], __builtin_fmaf(v.y, f[1], __builtin_fmaf(v.x, f[0], g)));
+                        }
+                    }
+                    jgrad[j] = g;
+                }
+                wave_sync();
+                for (int d = lane; d < a.dim; d += kWave) {
+                    float gd = 0.0f;
+                    for (int j = 0; j < nj; ++j) {
+                        const JointRec& J = M.joints[j];
+                        if (J.mimic >= 0) {
+                            if (J.mimic == d) gd = __builtin_fmaf(J.mul, jgrad[j], gd);
+                        } else if (J.act == d) {
+                            gd += jgrad[j];
+                        }
+                    }
+                    a.out_grad[static_cast<size_t>(idx) * a.dim + d] = gd;
+                }
+                wave_sync();
+            }
+        }
+        wave_sync();
+    }
+    if (lane == 0 && a.flops && flops) global_add(a.flops, flops);
+}
+
+// Jacobi step of every interior waypoint: candidate = clamp(q - step * (dC/dq + dS/dq)), with
+// S = w * sum_k |q_{k+1} - q_k|^2, so dS/dq_k = 2 w (2 q_k - q_{k-1} - q_{k+1}); end points stay.
+__global__ void refine_update_kernel(const float* paths, const float* grad, int P, int L, int dim, float step,
+                                     float smooth, const float* lo, const float* hi, float* cand) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long total = static_cast<long long>(P) * L * dim;
+    if (t >= total) return;
+    const int d = static_cast<int>(t % dim);
+    const int k = static_cast<int>((t / dim) % L);
+    const float q = paths[t];
+    if (k == 0 || k == L - 1) {
+        cand[t] = q;
+        return;
+    }
+    const float qm = paths[t - dim], qp = paths[t + dim];
+    const float gs = 2.0f * smooth * ((2.0f * q - qm) - qp);
+    float c = q - step * (grad[t] + gs);
+    if (lo && hi) c = fminf(fmaxf(c, lo[d]), hi[d]);
+    cand[t] = c;
+}
+
+// Keep a candidate waypoint only where the exact predicate found it free.
+__global__ void refine_accept_kernel(float* paths, const float* cand, const uint8_t* hit, int n, int dim) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(n) * dim) return;
+    if (!hit[t / dim]) paths[t] = cand[t];
+}
+
+// Total cost of all paths: sum of the per-waypoint collision costs plus the smoothness term.
+// One work-group, fixed summation order (deterministic).
+__global__ __launch_bounds__(1024) void refine_total_kernel(const float* paths, const float* cost, int P, int L,
+                                                            int dim, float smooth, float* out) {
+    __shared__ float red[1024];
+    const int tid = threadIdx.x;
+    float s = 0.0f;
+    const long long n = static_cast<long long>(P) * L;
+    for (long long i = tid; i < n; i += blockDim.x) {
+        s += cost[i];
+        const int k = static_cast<int>(i % L);
+        if (k + 1 < L) {
+            float e = 0.0f;
+            for (int d = 0; d < dim; ++d) {
+                const float df = paths[(i + 1) * dim + d] - paths[i * dim + d];
+                e = __builtin_fmaf(df, df, e);
+            }
+            s = __builtin_fmaf(smooth, e, s);
+        }
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) *out = red[0];
+}
+
+struct CostPlan {
+    CostArgs a;
+    size_t lds;
+    int grid, nt;
+};
+
+int plan_cost(const prrtc_robot* robot, const float* world_spheres, int nws, int n, int dim, float beta,
+              CostPlan& cp) {
+    int rc = check_robot(robot);
+    if (rc) return rc;
+    if (n < 0 || dim < 1 || dim > PRRTC_MAX_DIM) return fail_with(PRRTC_E_INVALID, "bad n or dim");
+    if (nws < 0 || (nws > 0 && !world_spheres)) return fail_with(PRRTC_E_INVALID, "bad world spheres");
+    if (!(beta > 0.0f)) return fail_with(PRRTC_E_INVALID, "beta must be > 0");
+    if (robot->n_robot_spheres > 0 && (!robot->sphere_link_idx || !robot->sphere_local || !robot->sphere_radius))
+        return fail_with(PRRTC_E_INVALID, "robot sphere arrays are null");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail_with(PRRTC_E_HIP, "hipGetDevice failed");
+    const DevInfo& di = dev_info(dev);
+    prrtc_world w;
+    std::memset(&w, 0, sizeof(w));
+    w.world_spheres = world_spheres;
+    w.n_world_spheres = nws;
+    CostArgs& a = cp.a;
+    std::memset(&a, 0, sizeof(a));
+    int nt = 256;
+    const int nj = robot->n_joints, nrs = robot->n_robot_spheres;
+    a.acc_stride = static_cast<int>((7 * nrs + 7 * nj + 3) & ~3);
+    for (;;) {
+        a.ml = model_layout(nj, nrs, 0, nws, 0, 0, 0, nt / kWave, kFBP);
+        a.off_acc = align16(a.ml.end);
+        a.off_scan = align16(a.off_acc + sizeof(float) * a.acc_stride * (nt / kWave));
+        cp.lds = align16(a.off_scan + sizeof(int) * 20);
+        if (cp.lds <= static_cast<size_t>(di.max_lds) || nt <= kWave) break;
+        nt -= kWave;
+    }
+    if (cp.lds > static_cast<size_t>(di.max_lds))
+        return fail_with(PRRTC_E_LDS, "robot + world spheres exceed the work-group LDS");
+    a.robot = robot_dev(*robot);
+    a.world = world_dev(w, false);
+    a.n = n;
+    a.dim = dim;
+    a.beta = beta;
+    cp.nt = nt;
+    const int batches = (n + kFB - 1) / kFB;
+    cp.grid = std::max(1, std::min((batches + nt / kWave - 1) / (nt / kWave), 8 * di.cus));
+    return PRRTC_OK;
+}
+
+hipError_t launch_cost(const CostPlan& cp, hipStream_t s) {
+    const void* fn = cp.a.out_grad ? reinterpret_cast<const void*>(&sphere_cost_kernel<true>)
+                                   : reinterpret_cast<const void*>(&sphere_cost_kernel<false>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(cp.lds));
+    if (e != hipSuccess) return e;
+    if (cp.a.out_grad) hipLaunchKernelGGL(sphere_cost_kernel<true>, dim3(cp.grid), dim3(cp.nt), cp.lds, s, cp.a);
+    else hipLaunchKernelGGL(sphere_cost_kernel<false>, dim3(cp.grid), dim3(cp.nt), cp.lds, s, cp.a);
+    return hipGetLastError();
+}
+
+struct RefineWs {
+    size_t cand, grad, cost, hit, total;
+};
+
+RefineWs refine_ws(long long n, int dim) {
+    RefineWs w;
+    size_t o = 0;
+    w.cand = o; o = align16(o + sizeof(float) * n * dim);
+    w.grad = o; o = align16(o + sizeof(float) * n * dim);
+    w.cost = o; o = align16(o + sizeof(float) * n);
+    w.hit = o; o = align16(o + static_cast<size_t>(n));
+    w.total = o + 256;
+    return w;
+}
+
+}  // namespace prrtc
+
+using namespace prrtc;
+
+extern "C" {
+
+int prrtc_sphere_cost(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim, const prrtc_robot* robot,
+                      const float* world_spheres, int32_t n_world_spheres, float beta, float* out_cost,
+                      float* out_grad, uint64_t* work_flops) {
+    CostPlan cp;
+    int rc = plan_cost(robot, world_spheres, n_world_spheres, n, dim, beta, cp);
+    if (rc) return rc;
+    if (n == 0) return PRRTC_OK;
+    if (!configs || !out_cost) return fail_with(PRRTC_E_INVALID, "configs / out_cost is null");
+    cp.a.configs = configs;
+    cp.a.out_cost = out_cost;
+    cp.a.out_grad = out_grad;
+    cp.a.flops = reinterpret_cast<unsigned long long*>(work_flops);
+    const hipError_t e = launch_cost(cp, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    return PRRTC_OK;
+}
+
+size_t prrtc_refine_workspace_bytes(int32_t n_paths, int32_t n_waypoints, int32_t dim) {
+    if (n_paths < 0 || n_waypoints < 0 || dim < 1) return 0;
+    return refine_ws(static_cast<long long>(n_paths) * n_waypoints, dim).total;
+}
+
+int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int32_t n_waypoints, int32_t dim,
+                       const float* min_vals, const float* max_vals, const prrtc_robot* robot,
+                       const prrtc_world* world, int32_t collision_mode, float collision_margin, float beta,
+                       float smooth_weight, float step, int32_t iterations, float* out_cost, void* workspace,
+                       size_t workspace_bytes, uint64_t* work_flops) {
+    if (n_paths < 0 || n_waypoints < 2 || iterations < 0)
+        return fail_with(PRRTC_E_INVALID, "need n_paths >= 0, n_waypoints >= 2, iterations >= 0");
+    if (!world) return fail_with(PRRTC_E_INVALID, "world is null");
+    if (!(step >= 0.0f) || !(smooth_weight >= 0.0f)) return fail_with(PRRTC_E_INVALID, "step and weight must be >= 0");
+    if ((min_vals == nullptr) != (max_vals == nullptr)) return fail_with(PRRTC_E_INVALID, "give both limits or none");
+    int rc = check_world(world);
+    if (rc) return rc;
+    const long long n = static_cast<long long>(n_paths) * n_waypoints;
+    if (n > (1ll << 30)) return fail_with(PRRTC_E_INVALID, "too many waypoints");
+    CostPlan cp;
+    rc = plan_cost(robot, world->world_spheres, world->n_world_spheres, static_cast<int>(n), dim, beta, cp);
+    if (rc) return rc;
+    if (n == 0) return PRRTC_OK;
+    if (!paths) return fail_with(PRRTC_E_INVALID, "paths is null");
+    const RefineWs wl = refine_ws(n, dim);
+    if (!workspace || workspace_bytes < wl.total) return fail_with(PRRTC_E_WORKSPACE, "workspace too small");
+    char* ws = static_cast<char*>(workspace);
+    float* cand = reinterpret_cast<float*>(ws + wl.cand);
+    float* grad = reinterpret_cast<float*>(ws + wl.grad);
+    float* cost = reinterpret_cast<float*>(ws + wl.cost);
+    uint8_t* hit = reinterpret_cast<uint8_t*>(ws + wl.hit);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    cp.a.configs = paths;
+    cp.a.out_cost = cost;
+    cp.a.flops = reinterpret_cast<unsigned long long*>(work_flops);
+    const long long elems = n * dim;
+    const int eblocks = static_cast<int>((elems + 255) / 256);
+    for (int it = 0; it <= iterations; ++it) {
+        const bool last = it == iterations;
+        cp.a.out_grad = last ? nullptr : grad;
+        hipError_t e = launch_cost(cp, s);
+        if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+        if (out_cost) {
+            hipLaunchKernelGGL(refine_total_kernel, dim3(1), dim3(1024), 0, s, paths, cost, n_paths, n_waypoints, dim,
+                               smooth_weight, out_cost + it);
+        }
+        if (last) break;
+        hipLaunchKernelGGL(refine_update_kernel, dim3(eblocks), dim3(256), 0, s, paths, grad, n_paths, n_waypoints,
+                           dim, step, smooth_weight, min_vals, max_vals, cand);
+        rc = prrtc_check_configs(stream, cand, static_cast<int32_t>(n), dim, robot, world, collision_mode,
+                                 collision_margin, hit, nullptr, nullptr, work_flops);
+        if (rc) return rc;
+        hipLaunchKernelGGL(refine_accept_kernel, dim3(eblocks), dim3(256), 0, s, paths, cand, hit,
+                           static_cast<int>(n), dim);
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+    }
+    return PRRTC_OK;
+}
+
+}  // extern "C"

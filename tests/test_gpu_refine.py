@@ -1,0 +1,131 @@
+"""GPU: the sphere-penetration cost / adjoint kernels and path refinement against the FP64 oracle.
+
+The cost and its gradient are floating-point results of a different summation order than the
+oracle's, so they are compared within tolerances (stated per test): FP32 kernels vs FP64
+analytic gradient, FP64 central differences (h = 1e-4) and torch float64 autograd.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from prrax_amd import scenes
+from prrax_amd.prrtc import prrtc_plan
+from prrax_amd.refine import SphereCost, refine_paths, resample_path, sphere_cost
+
+from helpers import query, torch_sphere_cost64, with_prismatic_and_screw
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _configs(cc, lo, hi, n, seed, min_cost=1e-3):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        q = rng.uniform(lo, hi)
+        if O.sphere_cost64(cc, q) > min_cost:
+            out.append(q)
+    return np.array(out, np.float32)
+
+
+def _cases():
+    c5 = scenes.scene_c5()
+    c4 = scenes.scene_c4()
+    return {
+        "c5": (c5.context(), c5.robot),
+        "c4": (c4.context(), c4.robot),
+        "c5_prism_screw": (with_prismatic_and_screw(c5.context()), c5.robot),
+    }
+
+
+@pytest.mark.parametrize("name", ["c5", "c4", "c5_prism_screw"])
+def test_cost_and_gradient_match_fp64(name):
+    cc, robot = _cases()[name]
+    say(f"sphere cost {name}")
+    qs = _configs(cc, robot.lower, robot.upper, 48, seed=5)
+    sc = SphereCost(cc, robot.dim)
+    cost, grad = sc(torch.from_numpy(qs).cuda(), grad=True)
+    cost, grad = cost.cpu().numpy(), grad.cpu().numpy()
+    h = 1e-4
+    for k, q in enumerate(qs.astype(np.float64)):
+        c64 = O.sphere_cost64(cc, q)
+        # FP32 sums of O(10^3) terms: relative 1e-4 of the cost, plus 1e-6 absolute
+        assert abs(cost[k] - c64) <= 1e-4 * abs(c64) + 1e-6, (k, cost[k], c64)
+        g64 = O.sphere_cost_grad64(cc, q)
+        scale = max(1.0, np.abs(g64).max())
+        # gradient: FP32 FK + accumulation vs FP64, 2e-3 of the largest component
+        assert np.abs(grad[k] - g64).max() <= 2e-3 * scale, (k, grad[k], g64)
+        if k < 8:
+            fd = np.array([(O.sphere_cost64(cc, q + h * e) - O.sphere_cost64(cc, q - h * e)) / (2 * h)
+                           for e in np.eye(q.shape[0])])
+            assert np.abs(grad[k] - fd).max() <= 2e-3 * scale, (k, grad[k], fd)
+
+
+def test_cost_matches_torch_autograd_and_custom_function():
+    cc, robot = _cases()["c5"]
+    qs = _configs(cc, robot.lower, robot.upper, 4, seed=9)
+    sc = SphereCost(cc, robot.dim)
+    qt = torch.from_numpy(qs).cuda().requires_grad_(True)
+    c = sphere_cost(qt, sc)
+    (c * torch.arange(1, 5, device="cuda", dtype=torch.float32)).sum().backward()  # weighted: checks gout
+    got = qt.grad.cpu().numpy()
+    for k in range(4):
+        q64 = torch.tensor(qs[k].astype(np.float64), requires_grad=True)
+        ref = torch_sphere_cost64(cc, q64)
+        ref.backward()
+        want = (k + 1) * q64.grad.numpy()
+        assert abs(float(c[k]) - float(ref)) <= 1e-4 * abs(float(ref)) + 1e-6
+        assert np.abs(got[k] - want).max() <= 2e-3 * max(1.0, np.abs(want).max()), (got[k], want)
+
+
+def test_no_world_spheres_gives_zero_cost():
+    cc, robot = _cases()["c5"]
+    cc = dict(cc)
+    cc["world_spheres"] = np.zeros((0, 4), np.float32)
+    sc = SphereCost(cc, robot.dim)
+    cost, grad = sc(torch.zeros((5, robot.dim), device="cuda"), grad=True)
+    assert float(cost.abs().max()) == 0.0 and float(grad.abs().max()) == 0.0
+
+
+def test_refine_paths_keeps_ends_and_validity():
+    sc5 = scenes.scene_c5()
+    cc, robot = sc5.context(), sc5.robot
+    paths = []
+    for qi in range(6):
+        s, g = query(sc5, 5, qi)
+        r = prrtc_plan(s, g, collision_context=cc, max_iterations=2000, max_nodes=16384, min_vals=robot.lower,
+                       max_vals=robot.upper)
+        if r.solved:
+            paths.append(resample_path(r.path, 64))
+    assert len(paths) >= 3
+    P = np.stack(paths)
+    say(f"refine {P.shape}")
+    out, costs = refine_paths(P, cc, iterations=50, step=2e-3, min_vals=robot.lower, max_vals=robot.upper)
+    out = out.cpu().numpy()
+    assert np.isfinite(costs).all()
+    assert costs[-1] < costs[0], costs
+    np.testing.assert_array_equal(out[:, 0], P[:, 0])
+    np.testing.assert_array_equal(out[:, -1], P[:, -1])
+    assert np.abs(out - P).max() > 0.0
+    ctx = O.Context(cc, robot.dim)
+    moved = np.any(out != P, axis=2)
+    hits = O.in_collision(ctx, out.reshape(-1, robot.dim)).reshape(out.shape[:2])
+    # every moved waypoint passed the exact predicate (the oracle's, bit-exact with the GPU's)
+    assert not np.any(hits & moved)
+    assert np.all(out >= robot.lower - 1e-6) and np.all(out <= robot.upper + 1e-6)
+
+
+def test_refine_zero_iterations_is_identity():
+    sc5 = scenes.scene_c5()
+    cc, robot = sc5.context(), sc5.robot
+    P = np.random.default_rng(0).uniform(robot.lower, robot.upper, size=(2, 5, robot.dim)).astype(np.float32)
+    out, costs = refine_paths(P, cc, iterations=0)
+    np.testing.assert_array_equal(out.cpu().numpy(), P)
+    assert costs.shape == (1,)
