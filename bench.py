@@ -90,6 +90,66 @@ def copy_calibration(mib: int):
     log(f"copy calibration: {mib} MiB read + {mib} MiB written")
 
 
+def refine_leg(n_paths: int, waypoints: int = 64, steps: int = 50):
+    """Path refinement: n_paths planner paths of the 7-DOF arm in a 100-sphere world, resampled
+    to 64 waypoints, 50 gradient steps of the sphere-penetration cost (hand-written FK adjoint),
+    each moved waypoint kept only if the exact predicate passes."""
+    import torch
+
+    from prrax_amd import prrtc_plan_batch, scenes
+    from prrax_amd.prrtc import CollisionChecker
+    from prrax_amd.refine import SphereCost, refine_paths, resample_path
+
+    sc = scenes.scene_c5()
+    cc, robot = sc.context(), sc.robot
+    free = CollisionChecker(cc, robot.dim, collision_checker="differentiable", collision_margin=1e-5).is_free
+    starts, goals = [], []
+    for qi in range(n_paths):
+        sg = scenes.sample_free(np.random.default_rng(scenes.query_seed(5, qi)), robot, 2, free)
+        starts.append(sg[0])
+        goals.append(sg[1:])
+    res = prrtc_plan_batch(np.stack(starts), goals, collision_context=cc, max_iterations=10_000, max_nodes=16_384,
+                           min_vals=robot.lower, max_vals=robot.upper)
+    paths = [resample_path(r.path, waypoints) for r in res if r.solved]
+    P = np.stack(paths)
+    log(f"refine: {P.shape[0]}/{n_paths} planned paths, {waypoints} waypoints each")
+    flops = torch.zeros(1, dtype=torch.int64, device="cuda")
+    refine_paths(P, cc, iterations=1, step=2e-3, min_vals=robot.lower, max_vals=robot.upper)  # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    out, costs = refine_paths(P, cc, iterations=steps, step=2e-3, min_vals=robot.lower, max_vals=robot.upper,
+                              flops=flops)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    moved = float(np.mean(np.any(out.cpu().numpy() != P, axis=2)[:, 1:-1]))
+    # cost + adjoint kernel alone, all waypoints of all paths in one launch
+    skc = SphereCost(cc, robot.dim)
+    q = torch.from_numpy(P.reshape(-1, robot.dim)).cuda()
+    kfl = torch.zeros(1, dtype=torch.int64, device="cuda")
+    skc(q, grad=True)
+    torch.cuda.synchronize()
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        skc(q, grad=True, flops=kfl)
+    e1.record()
+    torch.cuda.synchronize()
+    kms = e0.elapsed_time(e1) / reps
+    kt = float(kfl.item()) / reps / (kms / 1000.0) / 1e12
+    out = {"workload": f"{P.shape[0]} planner paths x {waypoints} waypoints, 7-DOF arm (48 spheres), 100 world "
+                       f"spheres, {steps} steps, beta 50, smoothness 0.1",
+           "paths": int(P.shape[0]), "ms": round(ms, 3), "paths_per_s": round(P.shape[0] / (ms / 1000.0), 1),
+           "cost_first": round(float(costs[0]), 4), "cost_last": round(float(costs[-1]), 4),
+           "interior_waypoints_moved": round(moved, 4),
+           "cost_grad_kernel": {"configs": int(q.shape[0]), "ms": round(kms, 4),
+                                "cfg_per_s": round(q.shape[0] / (kms / 1000.0), 1), "tflops": round(kt, 3),
+                                "frac_of_fp32_peak": round(kt / PEAK_FP32_TFLOPS, 4)}}
+    log(f"refine: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +165,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--max-iterations", type=int, default=10_000)
     ap.add_argument("--team-max", type=int, default=0, help="largest work-group team per open query (0 = library default)")
+    ap.add_argument("--refine-paths", type=int, default=512, help="paths of the refinement leg (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -212,6 +273,7 @@ def main():
                       "tflops": round(fl / (ms / 1000.0) / 1e12, 3),
                       "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
         log(f"fkcc {mode}: {fkcc[mode]}")
+    refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
     default_workload = (args.queries == 1024 and args.samples == 128 and args.granularity == 16
                         and args.max_iterations == 10_000)
@@ -240,6 +302,7 @@ def main():
         "batched_over_per_query": round(plans_per_s / pq["plans_per_s"], 2) if pq else None,
         "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
                              "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
+        "refine": refine,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                      "traffic": TRAFFIC_BYTES_PER_STEP if default_workload else None,
