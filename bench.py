@@ -90,6 +90,55 @@ def copy_calibration(mib: int):
     log(f"copy calibration: {mib} MiB read + {mib} MiB written")
 
 
+def dual_arm_leg(n_queries: int):
+    """14-DOF dual arm with mimic grippers (128 spheres, ~2000 self pairs), 150 spheres + 12
+    boxes + floor, goals 1-4 per query, step 0.3: one batched launch vs per-query launches."""
+    import torch
+
+    from prrax_amd import prepare_batch, scenes
+    from prrax_amd.prrtc import CollisionChecker
+
+    sc = scenes.scene_c4()
+    cc, robot = sc.context(), sc.robot
+    free = CollisionChecker(cc, robot.dim, collision_checker="differentiable", collision_margin=1e-5).is_free
+    starts, goals = [], []
+    for qi in range(n_queries):
+        rng = np.random.default_rng(scenes.query_seed(4, qi))
+        ng = int(rng.integers(1, 5))
+        sg = scenes.sample_free(rng, robot, 1 + ng, free)
+        starts.append(sg[0])
+        goals.append(sg[1:])
+    kw = dict(step_size=0.3, num_new_samples=128, max_iterations=10_000, max_nodes=32_768, min_vals=robot.lower,
+              max_vals=robot.upper)
+    pb = prepare_batch(np.stack(starts), goals, collision_context=cc, **kw)
+    pb.enqueue()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pb.enqueue()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    solved = pb.bufs.host_summary()["solved"] == 1
+    singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=cc, **kw) for i in range(n_queries)]
+    singles[0].enqueue()
+    torch.cuda.synchronize()
+    times = []
+    for sp in singles:
+        t = time.perf_counter()
+        sp.enqueue()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
+    out = {"workload": f"{n_queries} queries, 14-DOF dual arm (128 spheres, mimic fingers), 150 spheres + 12 boxes "
+                       "+ floor, 1-4 goals, step 0.3, S 128, max_nodes 32768",
+           "batched_plans_per_s": round(n_queries / dt, 2), "batched_ms": round(dt * 1000, 2),
+           "solve_rate": round(float(solved.mean()), 4),
+           "per_query_plans_per_s": round(n_queries / sum(times), 2),
+           "per_query_p50_ms": round(1000 * float(np.median(times)), 3),
+           "identical_solved_flags": bool(np.array_equal(solved, solved_pq))}
+    log(f"dual arm: {out}")
+    return out
+
+
 def refine_leg(n_paths: int, waypoints: int = 64, steps: int = 50):
     """Path refinement: n_paths planner paths of the 7-DOF arm in a 100-sphere world, resampled
     to 64 waypoints, 50 gradient steps of the sphere-penetration cost (hand-written FK adjoint),
@@ -166,6 +215,7 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=10_000)
     ap.add_argument("--team-max", type=int, default=0, help="largest work-group team per open query (0 = library default)")
     ap.add_argument("--refine-paths", type=int, default=512, help="paths of the refinement leg (0 = skip)")
+    ap.add_argument("--dual-arm-queries", type=int, default=256, help="queries of the 14-DOF leg (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -274,6 +324,7 @@ def main():
                       "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
         log(f"fkcc {mode}: {fkcc[mode]}")
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
+    dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
     default_workload = (args.queries == 1024 and args.samples == 128 and args.granularity == 16
                         and args.max_iterations == 10_000)
@@ -303,6 +354,7 @@ def main():
         "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
                              "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
         "refine": refine,
+        "dual_arm": dual,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                      "traffic": TRAFFIC_BYTES_PER_STEP if default_workload else None,

@@ -192,6 +192,29 @@ def test_batch_uneven_matches_single_queries():
         compare_plan(single, ora, f"single q{qi}")
 
 
+def test_dual_arm_batch_bit_exact():
+    """14-DOF dual arm with mimic fingers (the dim > 8 kernels): a batch with uneven goal
+    counts, step 0.3, against the oracle; then the same queries on teams (no pass 1)."""
+    sc = scenes.scene_c4()
+    cc = sc.context()
+    starts, goals = [], []
+    for qi in range(4):
+        s, g = query(sc, 4, qi, n_goals=1 + qi % 4)
+        starts.append(s)
+        goals.append(g)
+    kw = dict(step_size=0.3, num_new_samples=64, max_iterations=2000, max_nodes=16384, min_vals=sc.robot.lower,
+              max_vals=sc.robot.upper)
+    say("plan c4 batch")
+    res = prrtc_plan_batch(np.stack(starts), goals, collision_context=cc, **kw)
+    say("plan c4 batch on teams")
+    res_t = prrtc_plan_batch(np.stack(starts), goals, collision_context=cc, pass1_rounds=0, team_max=16, **kw)
+    for qi in range(4):
+        ora = O.plan(O.Context(cc, 14), starts[qi], goals[qi], **kw)
+        compare_plan(res[qi], ora, f"c4 q{qi}")
+        compare_plan(res_t[qi], ora, f"c4 teams q{qi}")
+        assert res[qi].solved
+
+
 def test_paths_valid_under_fp64_checker():
     """Every returned path's edges are free at 4x the planner's waypoint resolution under the
     FP64 checker (configs within 1e-5 m of contact are excluded as undecidable)."""
