@@ -476,36 +476,50 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         }
         __syncthreads();
         // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku
-        // consecutive waypoints of one edge; any wave of the team claims units of any explorer
-        // whose edge is not yet known to collide (FK one per lane, then checks in order).
+        // consecutive waypoints of one edge (FK one per lane, then checks in order; a collision
+        // anywhere on the edge stops the remaining units of that edge).
         {
-            int nact = 0;
-            for (int e0 = 0; e0 < S; e0 += nt) nact += __syncthreads_count(e0 + tid < S && state[e0 + tid] == 1);
-            const int ku = unit_size(static_cast<long long>(nact) * G, W * nwaves);
+            const int nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
+            const int ku = unit_size(static_cast<long long>(nl) * G, W * nwaves);
             const int cps = (G + ku - 1) / ku;
+            // ku < kFB (more waves than 16-waypoint units): a fixed round-robin deal of small units;
+            // otherwise units are claimed dynamically (claim counter per explorer), which
+            // balances the load and skips edges already known to collide.
+            const bool deal = ku < kFB;
+            int us = m * nwaves + wid;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
-                int e = -1;
-                for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
-                    bool ok = false;
-                    int ex = 0;
-                    if (b0 + lane < S) {
-                        ex = start + b0 + lane;
-                        if (ex >= S) ex -= S;
-                        ok = state[ex] == 1 && gload(&V.g_fail[ex]) == 0 && gload(&V.g_eclaim[ex]) < cps;
+                int e = -1, k0 = 0;
+                if (deal) {
+                    if (us >= nl * cps) break;
+                    e = list[us / cps];
+                    k0 = (us - (us / cps) * cps) * ku;
+                    us += W * nwaves;
+                    if (uniform(gload(&V.g_fail[e]))) continue;
+                } else {
+                    for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
+                        bool ok = false;
+                        int ex = 0;
+                        if (b0 + lane < S) {
+                            ex = start + b0 + lane;
+                            if (ex >= S) ex -= S;
+                            ok = state[ex] == 1 && gload(&V.g_fail[ex]) == 0 && gload(&V.g_eclaim[ex]) < cps;
+                        }
+                        const unsigned long long mask = __ballot(ok);
+                        if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
                     }
-                    const unsigned long long mask = __ballot(ok);
-                    if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                    e = uniform(e);
+                    if (e < 0) break;
+                    start = e;
+                    int u = 0;
+                    if (lane == 0)
+                        u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_eclaim[e], 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    u = uniform(__shfl(u, 0, kWave));
+                    if (u >= cps) continue;  // another wave took the last unit
+                    k0 = u * ku;
                 }
-                e = uniform(e);
-                if (e < 0) break;
-                start = e;
-                int u = 0;
-                if (lane == 0) u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_eclaim[e], 1, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
-                u = uniform(__shfl(u, 0, kWave));
-                if (u >= cps) continue;  // another wave took the last unit
-                const int k0 = u * ku, nk = min(ku, G - k0);
+                const int nk = min(ku, G - k0);
                 if (lane < nk) {
                     const float* T = side[e] ? TB : TA;
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
@@ -616,62 +630,96 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         }
         __syncthreads();
         // ---- phase 5: connect walks.  Work unit = ku consecutive waypoints of sub-edge j of one
-        // explorer's walk; waves claim units of any explorer (claim counter per explorer), and a
-        // collision at sub-edge j lowers the explorer's cap to j, so units past the first failing
-        // sub-edge stop being handed out.  g_cfail[e] ends as that first failing sub-edge.
+        // explorer's walk; a collision at sub-edge j lowers the explorer's first failing sub-edge
+        // g_cfail[e] (an order-independent minimum), and units past it are skipped.
         {
+            const int nl = compact(S, [&](int e) { return newi[e] >= 0; }, list, scan);
             int nsub = 0;
-            for (int e0 = 0; e0 < S; e0 += nt) {
-                const int e = e0 + tid;
-                const int v = (e < S && newi[e] >= 0) ? next[e] : 0;
+            for (int k0 = 0; k0 < nl; k0 += nt) {
+                const int k = k0 + tid;
                 int tot;
-                block_excl_scan(v, scan, &tot);
+                block_excl_scan(k < nl ? next[list[k]] : 0, scan, &tot);
                 nsub += tot;
             }
             const int ku = unit_size(static_cast<long long>(nsub) * G, W * nwaves);
             const int cps = (G + ku - 1) / ku;
+            // ku < kFB: a fixed round-robin deal (unit offsets of the walkers in aux0); otherwise
+            // units are claimed dynamically, and a collision at sub-edge j lowers the explorer's
+            // cap to j, so units past the first failure stop being handed out.
+            const bool deal = ku < kFB;
+            int total = 0;
+            if (deal) {
+                for (int k0 = 0; k0 < nl; k0 += nt) {
+                    const int k = k0 + tid;
+                    int tot;
+                    const int pre = block_excl_scan(k < nl ? next[list[k]] * cps : 0, scan, &tot);
+                    if (k < nl) aux0[k] = total + pre;
+                    total += tot;
+                }
+                __syncthreads();
+            }
+            int us = m * nwaves + wid;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
-                int e = -1;
-                for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
-                    bool ok = false;
-                    int ex = 0;
-                    if (b0 + lane < S) {
-                        ex = start + b0 + lane;
-                        if (ex >= S) ex -= S;
-                        if (newi[ex] >= 0) {
-                            const int cap = min(next[ex], gload(&V.g_cfail[ex])) * cps;
-                            ok = gload(&V.g_claim[ex]) < cap;
-                        }
+                int e = -1, jj = 0, k0 = 0;
+                if (deal) {
+                    if (us >= total) break;
+                    int lo = 0, hi = nl - 1;  // the last walker whose offset is <= us
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (aux0[mid] <= us) lo = mid;
+                        else hi = mid - 1;
                     }
-                    const unsigned long long mask = __ballot(ok);
-                    if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                    e = list[lo];
+                    const int rr = us - aux0[lo];
+                    jj = rr / cps;
+                    k0 = (rr - jj * cps) * ku;
+                    us += W * nwaves;
+                } else {
+                    for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
+                        bool ok = false;
+                        int ex = 0;
+                        if (b0 + lane < S) {
+                            ex = start + b0 + lane;
+                            if (ex >= S) ex -= S;
+                            if (newi[ex] >= 0) {
+                                const int cap = min(next[ex], gload(&V.g_cfail[ex])) * cps;
+                                ok = gload(&V.g_claim[ex]) < cap;
+                            }
+                        }
+                        const unsigned long long mask = __ballot(ok);
+                        if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                    }
+                    e = uniform(e);
+                    if (e < 0) break;
+                    start = e;
+                    int u = 0;
+                    if (lane == 0)
+                        u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_claim[e], 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    u = uniform(__shfl(u, 0, kWave));
+                    jj = u / cps;
+                    k0 = (u - jj * cps) * ku;
+                    if (jj >= next[e]) continue;  // past the end of the walk
                 }
-                e = uniform(e);
-                if (e < 0) break;
-                start = e;
-                int u = 0;
-                if (lane == 0) u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_claim[e], 1, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
-                u = uniform(__shfl(u, 0, kWave));
-                const int j = u / cps, k0 = (u - j * cps) * ku, nk = min(ku, G - k0);
-                if (j >= min(next[e], uniform(gload(&V.g_cfail[e])))) continue;  // past the end or the first failure
+                if (jj >= uniform(gload(&V.g_cfail[e]))) continue;  // past the first failure
+                const int nk = min(ku, G - k0);
                 if (lane < nk) {
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
                     for (int d = 0; d < dim; ++d) {
                         float c = cand[e * dim + d];
                         const float v = vec[e * dim + d];
-                        for (int jj = 0; jj < j; ++jj) c += v;
+                        for (int q = 0; q < jj; ++q) c += v;
                         L.q[d * kFBP] = c + tf * v;
                     }
                     if (M.enabled) fk_lane(M, L);
                 }
                 wave_sync();
                 for (int c = 0; c < nk; ++c) {
-                    if (j >= uniform(gload(&V.g_cfail[e]))) break;
+                    if (jj >= uniform(gload(&V.g_cfail[e]))) break;
                     if (lane == 0) ++evals;
                     if (wave_in_collision(M, WA.T + c, kFBP, flops)) {
-                        if (lane == 0) gmin(&V.g_cfail[e], j);
+                        if (lane == 0) gmin(&V.g_cfail[e], jj);
                         break;
                     }
                 }
@@ -1093,7 +1141,9 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     const int slots = (launch && launch->slots > 0) ? std::min(launch->slots, resident) : resident;
     a.slots = std::max(1, std::min(slots, kMaxSlots));
     // A lone query goes straight to a team; a batch first gives every query a short solo run.
-    a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (B == 1 ? 0 : 24);
+    // Pass 1 (one work-group per query) pays off only when the batch fills the resident slots;
+    // a smaller batch starts directly on teams of >= 2 work-groups.
+    a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (2 * B <= a.slots ? 0 : 24);
     a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 256;
     a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 32;
     hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1);

@@ -30,10 +30,12 @@
 
 namespace prrtc {
 
-// Algorithmic flops (FMA = 2) per robot-world sphere pair: cost 17 (difference 3, squared
-// norm 5, sqrt, penetration 2, scale, exp, log1p, add, divide), gradient 11 more (sigmoid 3,
+// Algorithmic flops (FMA = 2) per robot-world sphere pair: distance test 12 (difference 3,
+// squared norm 5, sqrt, penetration 2, scale) for every pair; for pairs within reach (z >=
+// kFarZ) the softplus 5 more (exp, log1p, max, add, divide) and the gradient 11 more (sigmoid 3,
 // 1/d and scale 2, accumulate 6); per joint of the adjoint 30.
-constexpr unsigned kFlopsPairCost = 17, kFlopsPairGrad = 11, kFlopsJointAdj = 30;
+constexpr unsigned kFlopsPairDist = 12, kFlopsPairCost = 5, kFlopsPairGrad = 11, kFlopsJointAdj = 30;
+constexpr float kFarZ = -30.0f;
 
 struct CostArgs {
     RobotDev robot;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(256) void sphere_cost_kernel(CostArgs a) {
             const int idx = b * kFB + c;
             const float* T = W.T + c;
             float cost = 0.0f;
-            int active = 0;
+            int active = 0, near = 0;
             for (int s = lane; s < nrs; s += kWave) {
                 const int js = M.sphere_js[s];
                 float gx = 0.0f, gy = 0.0f, gz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
@@ -112,10 +114,15 @@ __global__ __launch_bounds__(256) void sphere_cost_kernel(CostArgs a) {
                         const float dx = cx - o.x, dy = cy - o.y, dz = cz - o.z;
                         const float d = __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
                         const float z = beta * ((sp.w + o.w) - d);
-                        // softplus(z) = max(z, 0) + log1p(exp(-|z|)), stable for either sign
-                        cost += (fmaxf(z, 0.0f) + log1pf(expf(-fabsf(z)))) * inv_beta;
+                        // A pair with z < kFarZ contributes below exp(kFarZ) ~ 1e-13 (cost and
+                        // derivative alike): skipped.
+                        if (z < kFarZ) continue;
+                        ++near;
+                        // softplus(z) = max(z, 0) + log1p(e), sigmoid(z) from the same e = exp(-|z|)
+                        const float e = expf(-fabsf(z));
+                        cost += (fmaxf(z, 0.0f) + log1pf(e)) * inv_beta;
                         if (GRAD && d > 0.0f) {
-                            const float sg = 1.0f / (1.0f + expf(-z));  // d softplus_b / dx
+                            const float sg = (z >= 0.0f ? 1.0f : e) / (1.0f + e);  // d softplus_b / dx
                             const float k = -sg / d;
                             gx = __builtin_fmaf(k, dx, gx);
                             gy = __builtin_fmaf(k, dy, gy);
@@ -136,11 +143,12 @@ __global__ __launch_bounds__(256) void sphere_cost_kernel(CostArgs a) {
             if (lane == 0) a.out_cost[idx] = cost;
             // flops of this config: FK, sphere transforms, pair terms (+ adjoint)
             const int act_total = wave_sum_int(active);
+            const int near_total = wave_sum_int(near);
             if (lane == 0) {
                 flops += static_cast<unsigned long long>(kFlopsJoint) * nj +
                          static_cast<unsigned long long>(act_total) *
-                             (kFlopsXf + static_cast<unsigned long long>(M.n_ws) *
-                                             (kFlopsPairCost + (GRAD ? kFlopsPairGrad : 0u)));
+                             (kFlopsXf + static_cast<unsigned long long>(M.n_ws) * kFlopsPairDist) +
+                         static_cast<unsigned long long>(near_total) * (kFlopsPairCost + (GRAD ? kFlopsPairGrad : 0u));
                 if (GRAD) flops += static_cast<unsigned long long>(kFlopsJointAdj) * nj;
             }
             if (GRAD) {
