@@ -119,7 +119,7 @@ typedef struct {
     int32_t pass1_rounds; /* planner: rounds a query runs on one work-group before it is handed to a team of
                              work-groups; -1 = auto (24 when the batch fills the resident work-groups,
                              else 0: every query starts on a team) */
-    int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (256) */
+    int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (512) */
     int32_t epoch_rounds; /* planner: rounds a query runs between two re-formings of the teams of the
                              team pass; 0 = auto (32) */
 } prrtc_launch;

@@ -482,10 +482,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const int nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
             const int ku = unit_size(static_cast<long long>(nl) * G, W * nwaves);
             const int cps = (G + ku - 1) / ku;
-            // ku < kFB (more waves than 16-waypoint units): a fixed round-robin deal of small units;
-            // otherwise units are claimed dynamically (claim counter per explorer), which
-            // balances the load and skips edges already known to collide.
-            const bool deal = ku < kFB;
+            // Fewer waypoints than waves: a fixed round-robin deal of one-waypoint units (no
+            // claiming; nothing is left to skip).  Otherwise units are claimed dynamically (claim
+            // counter per explorer), which balances the load and skips edges known to collide.
+            const bool deal = static_cast<long long>(nl) * G < W * nwaves;
             int us = m * nwaves + wid;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
@@ -643,10 +643,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
             const int ku = unit_size(static_cast<long long>(nsub) * G, W * nwaves);
             const int cps = (G + ku - 1) / ku;
-            // ku < kFB: a fixed round-robin deal (unit offsets of the walkers in aux0); otherwise
-            // units are claimed dynamically, and a collision at sub-edge j lowers the explorer's
-            // cap to j, so units past the first failure stop being handed out.
-            const bool deal = ku < kFB;
+            // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers in
+            // aux0).  Otherwise units are claimed dynamically, and a collision at sub-edge j lowers
+            // the explorer's cap to j, so units past the first failure stop being handed out.
+            const bool deal = static_cast<long long>(nsub) * G < W * nwaves;
             int total = 0;
             if (deal) {
                 for (int k0 = 0; k0 < nl; k0 += nt) {
@@ -1144,7 +1144,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     // Pass 1 (one work-group per query) pays off only when the batch fills the resident slots;
     // a smaller batch starts directly on teams of >= 2 work-groups.
     a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (2 * B <= a.slots ? 0 : 24);
-    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 256;
+    a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 512;
     a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 32;
     hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1);
     hipError_t e = hipGetLastError();
