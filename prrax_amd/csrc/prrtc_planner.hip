@@ -954,8 +954,9 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
 
 // Pass 2: teams of W work-groups resume the queries left open by pass 1, in epochs.  In each
 // epoch the open queries are dealt to teams (W = slots / open queries, a power of two, capped)
-// and run for at most `epoch_rounds` rounds; the queries still open then form the next epoch's
-// list, so the teams grow as queries finish.  A grid barrier separates the epochs.
+// and run for at most 4 << epoch rounds (capped at `epoch_rounds`); the queries still open then
+// form the next epoch's list, so the teams grow as queries finish.  A grid barrier separates
+// the epochs.
 template <int MAXD>
 __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
@@ -981,6 +982,9 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
             member = b % W;
         }
         const int nteams = a.slots / W;
+        // Epoch budgets grow 4, 8, 16, ... up to epoch_rounds: early epochs end soon, so the
+        // teams re-form (and grow) while most queries of a batch finish.
+        const int budget = (a.epoch_rounds > 0) ? min(a.epoch_rounds, 4 << min(ep, 20)) : -1;
         if (team_id < nteams && team_id < ncont) {
             unsigned* ts = a.tsync + 2 * team_id;
             Team team{W, member, ts, a.ctl + CTL_ERROR, 0u};
@@ -990,7 +994,7 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
             for (int k = team_id; k < ncont; k += nteams) {
                 const unsigned long long t0 = wall_clock64();
                 const QueryView V = query_view(a, a.cont[cur * a.B + k]);
-                run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], a.epoch_rounds > 0 ? a.epoch_rounds : -1, t0, 2,
+                run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], budget, t0, 2,
                                 a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
                 team_barrier(team);  // members leave the query together before the next one
             }
