@@ -129,7 +129,8 @@ size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nod
 
 /* Byte offset, inside that workspace, of 16 uint64 counters the last planner call left there:
  * [0] configurations FK+collision-checked, [1] lockstep rounds summed over queries,
- * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] reserved,
+ * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] launch shape chosen (bits 0-15
+ * resident work-groups, 16-31 threads per work-group, 32-47 pass-1 rounds, 48-63 team cap),
  * [4..9] / [10..15] device wall-clock ticks spent in the round phases of the per-query pass /
  * the team pass (sample+NN, extend check, commit, connect NN, connect walk, connect commit).
  * No reference counterpart (the reference reports only kernel_time_ms). */

@@ -1003,10 +1003,11 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     }
 }
 
-__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, int nsync) {
+__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, int nsync,
+                             unsigned long long shape) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < kCtl) ctl[t] = 0;
-    if (t < 16) stats[t] = 0ull;
+    if (t < 16) stats[t] = (t == 3) ? shape : 0ull;
     for (int i = t; i < 2 * nsync; i += gridDim.x * blockDim.x) tsync[i] = 0u;
 }
 
@@ -1150,7 +1151,11 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (2 * B <= a.slots ? 0 : 24);
     a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 512;
     a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 32;
-    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1);
+    const unsigned long long shape = static_cast<unsigned long long>(a.slots) |
+                                     (static_cast<unsigned long long>(nt) << 16) |
+                                     (static_cast<unsigned long long>(a.pass1_rounds & 0xffff) << 32) |
+                                     (static_cast<unsigned long long>(a.wmax & 0xffff) << 48);
+    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1, shape);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
     e = small ? launch_passes<8>(a, nt, lds, stream) : launch_passes<16>(a, nt, lds, stream);

@@ -131,6 +131,9 @@ class PlanBuffers:
         wall-clock ticks per round phase for the per-query pass and the team pass."""
         raw = self.workspace[self.stats_offset:self.stats_offset + 128].cpu().numpy().view(np.uint64)
         out = {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2])}
+        shape = int(raw[3])
+        out["launch"] = {"slots": shape & 0xFFFF, "threads": (shape >> 16) & 0xFFFF,
+                         "pass1_rounds": (shape >> 32) & 0xFFFF, "team_max": (shape >> 48) & 0xFFFF}
         out["pass1_ticks"] = {k: int(v) for k, v in zip(self.PHASES, raw[4:10])}
         out["pass2_ticks"] = {k: int(v) for k, v in zip(self.PHASES, raw[10:16])}
         return out
