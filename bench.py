@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--team-max", type=int, default=0, help="largest work-group team per open query (0 = library default)")
     ap.add_argument("--refine-paths", type=int, default=512, help="paths of the refinement leg (0 = skip)")
     ap.add_argument("--dual-arm-queries", type=int, default=256, help="queries of the 14-DOF leg (0 = skip)")
+    ap.add_argument("--no-autotune", action="store_true", help="use S = --samples and the library's launch defaults")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -246,12 +247,22 @@ def main():
     log(f"workload built in {time.perf_counter() - t0:.1f}s")
     kw = dict(max_iterations=args.max_iterations, step_size=0.5, num_new_samples=args.samples, granularity=args.granularity,
               min_vals=robot.lower, max_vals=robot.upper)
+    launch = dict(threads=args.threads, slots=args.slots, team_max=args.team_max)
+    tuned = None
+    if not args.no_autotune:
+        from prrax_amd.autotune import autotune
+
+        t0 = time.perf_counter()
+        base = {k: v for k, v in kw.items() if k != "num_new_samples"}
+        tuned = autotune(starts, goals, worlds, max_nodes_per_query=nodes, log=log, **base)
+        kw["num_new_samples"] = tuned.num_new_samples
+        launch.update({k: v for k, v in tuned.launch.items()})
+        log(f"autotune: chose S={tuned.num_new_samples} {tuned.launch} in {time.perf_counter() - t0:.1f}s")
     # Longest queries first (more obstacles -> more work): results do not depend on the order.
     order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
                        kind="stable")
     pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
-                       max_nodes_per_query=nodes, order=order, threads=args.threads, slots=args.slots,
-                       team_max=args.team_max, **kw)
+                       max_nodes_per_query=nodes, order=order, **launch, **kw)
     # ---- batched
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -279,7 +290,7 @@ def main():
     pq = None
     if n_pq > 0:
         singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=[worlds[i]], max_nodes=nodes[i],
-                                 threads=args.threads, **kw) for i in range(n_pq)]
+                                 threads=launch["threads"], **kw) for i in range(n_pq)]
         singles[0].enqueue()
         torch.cuda.synchronize()
         times = []
@@ -326,7 +337,7 @@ def main():
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
-    default_workload = (args.queries == 1024 and args.samples == 128 and args.granularity == 16
+    default_workload = (args.queries == 1024 and kw["num_new_samples"] == 128 and args.granularity == 16
                         and args.max_iterations == 10_000)
     out = {
         "metric": METRIC,
@@ -343,8 +354,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.queries} uneven 7-DOF queries (10-300 spheres, 0-24 boxes, floor, 1-8 goals, "
                                "max_nodes 8192-65536), one persistent launch",
-                   "queries": args.queries, "num_new_samples": args.samples, "granularity": args.granularity,
-                   "step_size": 0.5, "max_iterations": args.max_iterations, "threads": args.threads, "slots": args.slots},
+                   "queries": args.queries, "num_new_samples": kw["num_new_samples"], "granularity": args.granularity,
+                   "step_size": 0.5, "max_iterations": args.max_iterations, "launch": stats.get("launch"),
+                   "autotuned": tuned is not None},
         "batched": {"plans_per_s": round(plans_per_s, 2), "solve_rate": round(float(solved_b.mean()), 4),
                     "p50_ms_per_plan_device": round(float(np.median(summ["kernel_time_ms"])), 3),
                     "kernel_ms_per_step": round(kernel_ms, 3), "configs_checked": stats["configs"],
@@ -353,6 +365,9 @@ def main():
         "batched_over_per_query": round(plans_per_s / pq["plans_per_s"], 2) if pq else None,
         "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
                              "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
+        "autotune": None if tuned is None else {
+            "chosen": tuned.kwargs(), "sample_plans_per_s": round(tuned.plans_per_s, 1),
+            "candidates": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()} for r in tuned.table]},
         "refine": refine,
         "dual_arm": dual,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
