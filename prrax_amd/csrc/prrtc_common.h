@@ -39,8 +39,8 @@ struct WorldDev {
 // wq + w * kMaxDim * fbp floats (configurations, element c of dim d at [d * fbp + c]) and
 // wT + w * nj * 7 * fbp floats (joint poses, element c of slot k at [k * fbp + c]).
 struct ModelLayout {
-    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, wq, wT, end;
-    int fbp;
+    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, gc, gf, go, gm, wq, wT, wcull, end;
+    int fbp, cull_stride;
 };
 
 __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, int nws, int nwc, int nwb, int nwh,
@@ -55,8 +55,14 @@ __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, in
     L.wcap = o; o = align16(o + sizeof(CapsuleL) * nwc);
     L.wbox = o; o = align16(o + sizeof(BoxL) * nwb);
     L.whs = o; o = align16(o + sizeof(float4) * nwh);
+    L.gc = o; o = align16(o + sizeof(float4) * nj);
+    L.gf = o; o = align16(o + sizeof(int) * nj);
+    L.go = o; o = align16(o + sizeof(int) * (nj + 1));
+    L.gm = o; o = align16(o + sizeof(int) * nrs);
     L.wq = o; o = align16(o + sizeof(float) * kMaxDim * fbp * nwaves);
     L.wT = o; o = align16(o + sizeof(float) * 7 * nj * fbp * nwaves);
+    L.cull_stride = static_cast<int>(align16(static_cast<size_t>(cull_stride_bytes(nrs, nj))));
+    L.wcull = o; o = align16(o + static_cast<size_t>(L.cull_stride) * nwaves);
     L.end = o;
     L.fbp = fbp;
     return L;
@@ -203,7 +209,18 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
         o.a0x = b[3]; o.a0y = b[4]; o.a0z = b[5];
         o.a1x = b[6]; o.a1y = b[7]; o.a1z = b[8];
         o.a2x = b[9]; o.a2y = b[10]; o.a2z = b[11];
-        o.h0 = b[12]; o.h1 = b[13]; o.h2 = b[14]; o.pad = 0.0f;
+        o.h0 = b[12]; o.h1 = b[13]; o.h2 = b[14];
+        // orthonormal axes (Gram matrix within 1e-4 of the identity) make the box test a
+        // 1-Lipschitz distance, which the culled check relies on
+        const float g00 = o.a0x * o.a0x + o.a0y * o.a0y + o.a0z * o.a0z;
+        const float g11 = o.a1x * o.a1x + o.a1y * o.a1y + o.a1z * o.a1z;
+        const float g22 = o.a2x * o.a2x + o.a2y * o.a2y + o.a2z * o.a2z;
+        const float g01 = o.a0x * o.a1x + o.a0y * o.a1y + o.a0z * o.a1z;
+        const float g02 = o.a0x * o.a2x + o.a0y * o.a2y + o.a0z * o.a2z;
+        const float g12 = o.a1x * o.a2x + o.a1y * o.a2y + o.a1z * o.a2z;
+        const bool ortho = fabsf(g00 - 1.0f) < 1e-4f && fabsf(g11 - 1.0f) < 1e-4f && fabsf(g22 - 1.0f) < 1e-4f &&
+                           fabsf(g01) < 1e-4f && fabsf(g02) < 1e-4f && fabsf(g12) < 1e-4f;
+        o.ortho = ortho ? 1.0f : 0.0f;
         WB[k] = o;
     }
     for (int k = tid; k < nwh; k += nt) WH[k] = make_halfspace(wh + 6 * k);
@@ -223,6 +240,43 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
         if (ok) PR[base + pre] = make_int2(i, j);
         base += tot;
     }
+    // Culling groups: the robot spheres of each frame (frames in order), with a bounding sphere
+    // centred on their local AABB, radius max |p - c| + |r|.  Thread j < nj handles frame j.
+    float4* GC = reinterpret_cast<float4*>(lds + Lo.gc);
+    int* GF = reinterpret_cast<int*>(lds + Lo.gf);
+    int* GO = reinterpret_cast<int*>(lds + Lo.go);
+    int* GM = reinterpret_cast<int*>(lds + Lo.gm);
+    int cnt = 0;
+    if (tid < nj)
+        for (int s2 = 0; s2 < nrs; ++s2) cnt += (SJ[s2] == tid) ? 1 : 0;
+    int n_groups, n_members;
+    const int gid = block_excl_scan((tid < nj && cnt > 0) ? 1 : 0, scan_scratch, &n_groups);
+    const int goff = block_excl_scan(tid < nj ? cnt : 0, scan_scratch, &n_members);
+    if (tid < nj && cnt > 0) {
+        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+        int w = goff;
+        for (int s2 = 0; s2 < nrs; ++s2) {
+            if (SJ[s2] != tid) continue;
+            GM[w++] = s2;
+            const float4 q = SP[s2];
+            lx = fminf(lx, q.x); ly = fminf(ly, q.y); lz = fminf(lz, q.z);
+            hx = fmaxf(hx, q.x); hy = fmaxf(hy, q.y); hz = fmaxf(hz, q.z);
+        }
+        const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
+        float rad = 0.0f;
+        for (int m = goff; m < w; ++m) {
+            const float4 q = SP[GM[m]];
+            const float dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+            rad = fmaxf(rad, __builtin_sqrtf(dx * dx + dy * dy + dz * dz) + fabsf(q.w));
+        }
+        GC[gid] = make_float4(cx, cy, cz, rad);
+        GF[gid] = tid;
+        GO[gid] = goff;
+    }
+    if (tid == 0) GO[n_groups] = n_members;
+    M.gcoarse = GC; M.gframe = GF; M.goff = GO; M.gmem = GM; M.n_groups = n_groups;
+    M.cull_base = lds + Lo.wcull;
+    M.cull_stride = Lo.cull_stride;
     M.joints = J; M.spheres = SP; M.sphere_js = SJ; M.pairs = PR;
     M.wsph = WS; M.wcap = WC; M.wbox = WB; M.whs = WH;
     M.n_joints = nj; M.n_spheres = nrs; M.n_pairs = base;
@@ -247,11 +301,6 @@ __device__ __forceinline__ WaveArea wave_area(char* lds, const ModelLayout& Lo, 
     return a;
 }
 
-// Orders this wave's LDS writes before its later LDS reads from other lanes.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

@@ -113,7 +113,8 @@ __device__ __forceinline__ void joint_prep(const float* tw, JointRec& j) {
 // World primitives in LDS, in the form the tests consume (derived values use the same
 // operation sequence as the oracle's inline computation).
 struct CapsuleL { float p1x, p1y, p1z, ax, ay, az, inv, r; };
-struct BoxL { float cx, cy, cz, a0x, a0y, a0z, a1x, a1y, a1z, a2x, a2y, a2z, h0, h1, h2, pad; };
+// `ortho` = 1 when the axes are orthonormal (within 1e-4): only such boxes may be culled.
+struct BoxL { float cx, cy, cz, a0x, a0y, a0z, a1x, a1y, a1z, a2x, a2y, a2z, h0, h1, h2, ortho; };
 
 __device__ __forceinline__ CapsuleL make_capsule(const float* c) {
     CapsuleL o;
@@ -144,6 +145,14 @@ struct Model {
     int enabled;
     int mode;       // 0 binary (also the coarse modes: same verdict), 1 margin
     float margin;
+    // bounding sphere of each frame's robot spheres (groups in frame order)
+    const float4* gcoarse;   // frame-local centre, radius = max(|p - c| + |r|)
+    const int* gframe;       // sorted-joint index of the group's frame
+    const int* goff;         // members of group g: gmem[goff[g] .. goff[g + 1])
+    const int* gmem;
+    int n_groups;
+    char* cull_base;         // per-wave cull areas (see wave_culled)
+    int cull_stride;
 };
 
 // ---------------------------------------------------------------------------------- tests
@@ -321,6 +330,12 @@ __device__ __forceinline__ void frame_from(const float* T, int ts, int js, Mat3&
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// Orders this wave's LDS writes before its later LDS reads from other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <bool MARGIN>
 __device__ __forceinline__ float wave_check(const Model& M, const float* T, int ts, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -418,12 +433,194 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
     return best;
 }
 
-// The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform.
+// ---------------------------------------------------------------------------------- culled checks
+// The same verdicts as wave_check, with less work: each frame's robot spheres are enclosed in a
+// bounding sphere (centre c, radius R = max |p - c| + |r| over its spheres), and a fine test of
+// obstacle o against the spheres of group g runs only if the bounding sphere does not prove them
+// all clear.  Proof: every fine test is a 1-Lipschitz distance of the sphere centre (point,
+// segment, orthonormal box, unit-normal halfspace), so for every member p,
+//   dist(p, o) >= dist(c, o) - |p - c| >= dist(c, o) - R + |r|,
+// and a coarse distance above R + |r_o| + cut + kCullPad rules out the squared-form hit
+// (d <= |r + r_o|) and, in margin mode (cut = margin), any distance below the margin.  kCullPad
+// (1 mm) covers the FP32 rounding of both transforms; boxes whose axes are not orthonormal and
+// halfspaces whose normal is not unit are never culled.  Every fine test that does run is the
+// flat check's own test on the same FP32 values, so the verdict is identical bit for bit.
+constexpr float kCullPad = 1e-3f;
+constexpr int kCullQ = 2 * kWave;  // refinement queue (obstacle * 64 + group) per wave
+
+__host__ __device__ inline int cull_stride_bytes(int nrs, int nj) {
+    return static_cast<int>(sizeof(float4)) * (nrs + nj) + static_cast<int>(sizeof(int)) * kCullQ;
+}
+
+__device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4 c, float lim) {
+    if (o < M.n_ws) {
+        const float4 w = M.wsph[o];
+        const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+        return __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) > lim + fabsf(w.w);
+    }
+    o -= M.n_ws;
+    if (o < M.n_wc) {
+        const CapsuleL& k = M.wcap[o];
+        return __builtin_sqrtf(capsule_d2(c.x, c.y, c.z, k)) > lim + fabsf(k.r);
+    }
+    o -= M.n_wc;
+    if (o < M.n_wb) {
+        const BoxL& b = M.wbox[o];
+        if (b.ortho == 0.0f) return false;
+        float q0, q1, q2;
+        box_q(c.x, c.y, c.z, b, q0, q1, q2);
+        const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
+        return __builtin_sqrtf(__builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0))) > lim;
+    }
+    o -= M.n_wb;
+    const float4 h = M.whs[o];
+    const float n2 = __builtin_fmaf(h.z, h.z, __builtin_fmaf(h.y, h.y, h.x * h.x));
+    if (fabsf(n2 - 1.0f) > 2e-4f) return false;
+    return __builtin_fmaf(h.z, c.z, __builtin_fmaf(h.y, c.y, h.x * c.x)) - h.w > lim;
+}
+
+// Fine test of robot sphere q (world x, y, z, r) against obstacle o: hit (binary) or distance
+// below the margin (MARGIN), exactly as the flat check computes it.
+template <bool MARGIN>
+__device__ __forceinline__ bool fine_test(const Model& M, int o, const float4 q, unsigned& fl) {
+    if (o < M.n_ws) {
+        fl += kFlopsSS;
+        return MARGIN ? dist_sphere(q.x, q.y, q.z, q.w, M.wsph[o]) < M.margin : hit_sphere(q.x, q.y, q.z, q.w, M.wsph[o]);
+    }
+    o -= M.n_ws;
+    if (o < M.n_wc) {
+        fl += kFlopsSC;
+        return MARGIN ? dist_capsule(q.x, q.y, q.z, q.w, M.wcap[o]) < M.margin
+                      : hit_capsule(q.x, q.y, q.z, q.w, M.wcap[o]);
+    }
+    o -= M.n_wc;
+    if (o < M.n_wb) {
+        fl += kFlopsSB;
+        return MARGIN ? dist_box(q.x, q.y, q.z, q.w, M.wbox[o]) < M.margin : hit_box(q.x, q.y, q.z, q.w, M.wbox[o]);
+    }
+    o -= M.n_wb;
+    fl += kFlopsSH;
+    return MARGIN ? dist_halfspace(q.x, q.y, q.z, q.w, M.whs[o]) < M.margin
+                  : hit_halfspace(q.x, q.y, q.z, q.w, M.whs[o]);
+}
+
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Number of coarse pairs p in [p0, p1) whose obstacle index o = p / ng lies in [lo, hi).
+__device__ __forceinline__ unsigned long long pairs_in(long long p0, long long p1, int ng, int lo, int hi) {
+    const long long a = max(p0, static_cast<long long>(lo) * ng), b = min(p1, static_cast<long long>(hi) * ng);
+    return b > a ? static_cast<unsigned long long>(b - a) : 0ull;
+}
+
+template <bool MARGIN>
+__device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
+    const int lane = threadIdx.x & (kWave - 1);
+    char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
+    float4* FP = reinterpret_cast<float4*>(area);          // robot spheres, world frame
+    float4* CW = FP + M.n_spheres;                          // group bounding spheres, world frame
+    int* Q = reinterpret_cast<int*>(CW + M.n_joints);       // refinement queue
+    const int ng = M.n_groups, nrs = M.n_spheres;
+    const float cut = MARGIN ? M.margin : 0.0f;
+    unsigned lf = 0;  // this lane's flops of fine tests
+    auto done = [&](bool v) {
+        flops += wave_sum_u32(lf);
+        return v;
+    };
+    for (int s0 = 0; s0 < nrs; s0 += kWave) {
+        const int s = s0 + lane;
+        bool act = false;
+        if (s < nrs) {
+            const int js = M.sphere_js[s];
+            float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (js >= 0) {
+                Mat3 R;
+                float tx, ty, tz;
+                frame_from(T, ts, js, R, tx, ty, tz);
+                const float4 sp = M.spheres[s];
+                const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+                o = make_float4(p.x, p.y, p.z, sp.w);
+                act = true;
+            }
+            FP[s] = o;
+        }
+        flops += static_cast<unsigned long long>(__popcll(__ballot(act))) * kFlopsXf;
+    }
+    for (int g = lane; g < ng; g += kWave) {
+        const float4 c = M.gcoarse[g];
+        Mat3 R;
+        float tx, ty, tz;
+        frame_from(T, ts, M.gframe[g], R, tx, ty, tz);
+        const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
+        CW[g] = make_float4(p.x, p.y, p.z, c.w);
+    }
+    flops += static_cast<unsigned long long>(ng) * kFlopsXf;
+    wave_sync();
+    const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
+    const int no = nws + nwc + nwb + nwh;
+    const long long total = static_cast<long long>(no) * ng;
+    int qn = 0;
+    auto refine = [&]() -> bool {
+        wave_sync();
+        bool h = false;
+        for (int e = lane; e < qn && !h; e += kWave) {
+            const int o = Q[e] >> 6, g = Q[e] & 63;
+            for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
+                if (fine_test<MARGIN>(M, o, FP[M.gmem[m]], lf)) {
+                    h = true;
+                    break;
+                }
+            }
+        }
+        qn = 0;
+        const bool any = wave_any(h);
+        wave_sync();
+        return any;
+    };
+    for (long long p0 = 0; p0 < total; p0 += kWave) {
+        const long long p = p0 + lane;
+        bool dirty = false;
+        int entry = 0;
+        if (p < total) {
+            const int o = static_cast<int>(p / ng), g = static_cast<int>(p - static_cast<long long>(o) * ng);
+            const float4 c = CW[g];
+            dirty = !coarse_clear(M, o, c, c.w + cut + kCullPad);
+            entry = (o << 6) | g;
+        }
+        const long long p1 = min(p0 + kWave, total);
+        flops += pairs_in(p0, p1, ng, 0, nws) * kFlopsSS + pairs_in(p0, p1, ng, nws, nws + nwc) * kFlopsSC +
+                 pairs_in(p0, p1, ng, nws + nwc, nws + nwc + nwb) * kFlopsSB +
+                 pairs_in(p0, p1, ng, nws + nwc + nwb, no) * kFlopsSH;
+        const unsigned long long mask = __ballot(dirty);
+        if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
+        qn += static_cast<int>(__popcll(mask));
+        if (qn >= kWave && refine()) return done(true);
+    }
+    if (qn > 0 && refine()) return done(true);
+    for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
+        const int k = p0 + lane;
+        bool h = false;
+        if (k < M.n_pairs) {
+            const int2 pr = M.pairs[k];
+            const float4 a = FP[pr.x], b = FP[pr.y];
+            h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+        }
+        flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
+        if (wave_any(h)) return done(true);
+    }
+    return done(false);
+}
+
+// The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform; the
+// culled check gives the flat check's verdict (see wave_culled).
 __device__ __forceinline__ bool wave_in_collision(const Model& M, const float* T, int ts, unsigned long long& flops) {
     if (!M.enabled) return false;
     flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
-    if (M.mode == 1) return wave_check<true>(M, T, ts, flops) < M.margin;
-    return wave_check<false>(M, T, ts, flops) != 0.0f;
+    if (M.mode == 1) return wave_culled<true>(M, T, ts, flops);
+    return wave_culled<false>(M, T, ts, flops);
 }
 
 }  // namespace prrtc

@@ -63,6 +63,41 @@ def test_collision_bit_exact(name, n, mode):
     assert 0.05 < hit.mean() < 0.999  # both outcomes well represented
 
 
+@pytest.mark.parametrize("mode", ["binary", "differentiable", "binary_coarse", "binary_coarse_coop"])
+def test_collision_degenerate_geometry_bit_exact(mode):
+    """Verdicts on worlds the coarse culling must not prune: negative radii (squared-form
+    tests), non-orthonormal boxes, non-unit halfspace normals, degenerate capsules; a tree robot
+    with prismatic, screw and mimic joints; configurations concentrated near contact."""
+    from test_oracle_fk_collision import random_tree_robot
+
+    cc, dim = random_tree_robot(3)
+    rng = np.random.default_rng(21)
+    ws = np.concatenate([rng.uniform(-0.6, 0.6, (60, 3)), rng.uniform(-0.05, 0.12, (60, 1))], 1)
+    caps = np.concatenate([rng.uniform(-0.6, 0.6, (6, 6)), rng.uniform(-0.02, 0.05, (6, 1))], 1)
+    caps[0, 3:6] = caps[0, :3]  # zero-length capsule
+    boxes = np.zeros((6, 15))
+    boxes[:, :3] = rng.uniform(-0.6, 0.6, (6, 3))
+    for k in range(6):
+        A = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        if k % 2:
+            A = A * rng.uniform(0.5, 1.5, size=3) + 0.1 * rng.normal(size=(3, 3))  # skewed, not orthonormal
+        boxes[k, 3:12] = A.T.reshape(-1)
+        boxes[k, 12:] = rng.uniform(-0.02, 0.2, 3)
+    hs = np.array([[0, 0, 1.3, 0, 0, -0.9], [0.2, 0.9, 0.1, 0, 0.8, 0]], np.float64)
+    cc.update(world_spheres=ws.astype(np.float32), world_capsules=caps.astype(np.float32),
+              world_boxes=boxes.astype(np.float32), world_halfspaces=hs.astype(np.float32))
+    q = rng.uniform(-2.8, 2.8, size=(60_000, dim)).astype(np.float32)
+    flat = O.Context(cc, dim, collision_mode=1, collision_margin=0.0)
+    md = O.min_distance(flat, q)
+    q = q[np.argsort(np.abs(md))[:20_000]]  # the 20k configurations closest to contact
+    say(f"degenerate geometry {mode}")
+    chk = CollisionChecker(cc, dim, collision_checker=mode, collision_margin=0.01)
+    hit = chk.in_collision(q).cpu().numpy()
+    ctx = O.Context(cc, dim, collision_mode=1 if mode == "differentiable" else 0, collision_margin=0.01)
+    np.testing.assert_array_equal(hit, O.in_collision(ctx, q))
+    assert 0.05 < hit.mean() < 0.95
+
+
 def test_nearest_neighbor_ties_and_parity():
     rng = np.random.default_rng(3)
     for dim, n, nq in ((7, 1000, 1), (7, 16384, 32), (14, 4096, 512), (3, 77, 40)):

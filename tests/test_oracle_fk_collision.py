@@ -246,3 +246,62 @@ def test_disabled_context_is_collision_free():
     cc = _single_sphere_robot({})
     ctx = O.Context(cc, 1)
     assert not O.in_collision(ctx, np.zeros(1, np.float32))[0]
+
+
+def _frame_groups(cc, n_frames):
+    """Per frame: member sphere indices, local AABB-centre bounding sphere (radius + |r|)."""
+    inv = np.asarray(cc["fk_topo_inv"])
+    links = np.asarray(cc["sphere_link_idx"])
+    loc = np.asarray(cc["sphere_local"], np.float32)
+    rad = np.asarray(cc["sphere_radius"], np.float32)
+    groups = []
+    for j in range(n_frames):
+        mem = [s for s, l in enumerate(links) if 0 <= l < n_frames and inv[l] == j]
+        if not mem:
+            continue
+        p = loc[mem]
+        c = (0.5 * (p.min(0) + p.max(0))).astype(np.float32)
+        R = np.float32(np.max(np.linalg.norm(p - c, axis=1) + np.abs(rad[mem])))
+        groups.append((j, mem, c, R))
+    return groups
+
+
+@pytest.mark.parametrize("name", ["c2", "tree"])
+def test_coarse_culling_never_drops_a_hit(name):
+    """The bound the culled GPU check relies on, in float32: whenever a frame's bounding sphere
+    is farther from an obstacle than R + |r_o| + 1 mm, no sphere of that frame hits it (binary,
+    squared form) nor comes within the margin."""
+    cc, dim = _robots()[name]
+    if name == "tree":
+        cc = dict(cc)
+        rng0 = np.random.default_rng(5)
+        cc["world_spheres"] = np.concatenate([rng0.uniform(-0.5, 0.5, (40, 3)), rng0.uniform(0.01, 0.1, (40, 1))],
+                                             1).astype(np.float32)
+    ctx = O.Context(cc, dim)
+    inv = np.asarray(cc["fk_topo_inv"])
+    n = np.asarray(cc["fk_parent_idx"]).shape[0]
+    groups = _frame_groups(cc, n)
+    ws = np.asarray(cc["world_spheres"], np.float32).reshape(-1, 4)
+    rad = np.asarray(cc["sphere_radius"], np.float32)
+    pad, margin = np.float32(1e-3), np.float32(0.02)
+    rng = np.random.default_rng(1)
+    lo, hi = (-2.8, 2.8)
+    culled = kept = 0
+    for _ in range(120):
+        q = rng.uniform(lo, hi, size=dim).astype(np.float32)
+        T = O.fk(ctx, q)                     # [n_j, 7] FP32, original joint order
+        P = O.sphere_positions(ctx, q)       # [n_rs, 3] FP32
+        for j, mem, c, R in groups:
+            fr = T[np.flatnonzero(inv == j)[0]]
+            Rm = O.quat_mat64(fr[:4]).astype(np.float32)
+            cw = (Rm @ c + fr[4:]).astype(np.float32)
+            d = np.linalg.norm(ws[:, :3] - cw, axis=1).astype(np.float32)
+            for mode_cut in (np.float32(0.0), margin):
+                cull = d > R + np.abs(ws[:, 3]) + mode_cut + pad
+                for o in np.flatnonzero(cull):
+                    dd = np.linalg.norm(P[mem] - ws[o, :3], axis=1)
+                    assert np.all(dd > np.abs(rad[mem] + ws[o, 3])), (j, o)          # no squared-form hit
+                    assert np.all(dd - rad[mem] - ws[o, 3] >= mode_cut), (j, o)     # no distance < cut
+                culled += int(cull.sum())
+                kept += int((~cull).sum())
+    assert culled > kept  # the bound prunes most pairs
