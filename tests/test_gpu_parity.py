@@ -83,7 +83,7 @@ def test_collision_degenerate_geometry_bit_exact(mode):
             A = A * rng.uniform(0.5, 1.5, size=3) + 0.1 * rng.normal(size=(3, 3))  # skewed, not orthonormal
         boxes[k, 3:12] = A.T.reshape(-1)
         boxes[k, 12:] = rng.uniform(-0.02, 0.2, 3)
-    hs = np.array([[0, 0, 1.3, 0, 0, -0.9], [0.2, 0.9, 0.1, 0, 0.8, 0]], np.float64)
+    hs = np.array([[0, 0, 1.3, 0, 0, -0.9], [0.2, 0.9, 0.1, 0, -1.2, 0]], np.float64)  # mostly on the free side
     cc.update(world_spheres=ws.astype(np.float32), world_capsules=caps.astype(np.float32),
               world_boxes=boxes.astype(np.float32), world_halfspaces=hs.astype(np.float32))
     q = rng.uniform(-2.8, 2.8, size=(60_000, dim)).astype(np.float32)
