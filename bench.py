@@ -314,26 +314,33 @@ def main():
     qcfg = torch.from_numpy(rng.uniform(sc2.robot.lower, sc2.robot.upper,
                                         size=(args.fkcc_configs, 7)).astype(np.float32)).cuda()
     fkcc = {}
-    for mode in (("binary", "differentiable") if args.fkcc_configs > 0 else ()):
+    # verdicts (binary; margin 0.02) use the coarse-culled check; the min-distance leg is the
+    # flat full sweep over every sphere pair (no early exit, no culling)
+    legs = (("binary", "binary", True), ("margin_verdict", "differentiable", True),
+            ("min_distance_full_sweep", "differentiable", False))
+    for label, mode, verdict in (legs if args.fkcc_configs > 0 else ()):
         chk = CollisionChecker(sc2.context(), 7, collision_checker=mode, collision_margin=0.02)
-        hit = torch.empty(args.fkcc_configs, dtype=torch.uint8, device="cuda")
+        hit = torch.empty(args.fkcc_configs, dtype=torch.uint8, device="cuda") if verdict else None
+        mind = None if verdict else torch.empty(args.fkcc_configs, dtype=torch.float32, device="cuda")
         flops = torch.zeros(1, dtype=torch.int64, device="cuda")
-        chk.enqueue(qcfg, hit, None)
+        chk.enqueue(qcfg, hit, mind)
         torch.cuda.synchronize()
         flops.zero_()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         reps = 3
         e0.record()
         for _ in range(reps):
-            chk.enqueue(qcfg, hit, None, flops)
+            chk.enqueue(qcfg, hit, mind, flops)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         fl = float(flops.item()) / reps
-        fkcc[mode] = {"cfg_per_s": round(args.fkcc_configs / (ms / 1000.0), 1), "ms": round(ms, 3),
-                      "tflops": round(fl / (ms / 1000.0) / 1e12, 3),
-                      "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
-        log(f"fkcc {mode}: {fkcc[mode]}")
+        fkcc[label] = {"cfg_per_s": round(args.fkcc_configs / (ms / 1000.0), 1), "ms": round(ms, 3),
+                       "tflops": round(fl / (ms / 1000.0) / 1e12, 3),
+                       "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
+        if verdict:
+            fkcc[label]["in_collision"] = round(float(hit.float().mean().item()), 4)
+        log(f"fkcc {label}: {fkcc[label]}")
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
