@@ -452,18 +452,16 @@ __host__ __device__ inline int cull_stride_bytes(int nrs, int nj) {
     return static_cast<int>(sizeof(float4)) * (nrs + nj) + static_cast<int>(sizeof(int)) * kCullQ;
 }
 
+// Obstacles of the culled check are numbered halfspaces, boxes, capsules, spheres: the few large
+// primitives (a floor, walls) come first, where a colliding configuration is found soonest.
 __device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4 c, float lim) {
-    if (o < M.n_ws) {
-        const float4 w = M.wsph[o];
-        const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
-        return __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) > lim + fabsf(w.w);
+    if (o < M.n_wh) {
+        const float4 h = M.whs[o];
+        const float n2 = __builtin_fmaf(h.z, h.z, __builtin_fmaf(h.y, h.y, h.x * h.x));
+        if (fabsf(n2 - 1.0f) > 2e-4f) return false;
+        return __builtin_fmaf(h.z, c.z, __builtin_fmaf(h.y, c.y, h.x * c.x)) - h.w > lim;
     }
-    o -= M.n_ws;
-    if (o < M.n_wc) {
-        const CapsuleL& k = M.wcap[o];
-        return __builtin_sqrtf(capsule_d2(c.x, c.y, c.z, k)) > lim + fabsf(k.r);
-    }
-    o -= M.n_wc;
+    o -= M.n_wh;
     if (o < M.n_wb) {
         const BoxL& b = M.wbox[o];
         if (b.ortho == 0.0f) return false;
@@ -473,35 +471,39 @@ __device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4
         return __builtin_sqrtf(__builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0))) > lim;
     }
     o -= M.n_wb;
-    const float4 h = M.whs[o];
-    const float n2 = __builtin_fmaf(h.z, h.z, __builtin_fmaf(h.y, h.y, h.x * h.x));
-    if (fabsf(n2 - 1.0f) > 2e-4f) return false;
-    return __builtin_fmaf(h.z, c.z, __builtin_fmaf(h.y, c.y, h.x * c.x)) - h.w > lim;
+    if (o < M.n_wc) {
+        const CapsuleL& k = M.wcap[o];
+        return __builtin_sqrtf(capsule_d2(c.x, c.y, c.z, k)) > lim + fabsf(k.r);
+    }
+    o -= M.n_wc;
+    const float4 w = M.wsph[o];
+    const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+    return __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) > lim + fabsf(w.w);
 }
 
 // Fine test of robot sphere q (world x, y, z, r) against obstacle o: hit (binary) or distance
 // below the margin (MARGIN), exactly as the flat check computes it.
 template <bool MARGIN>
 __device__ __forceinline__ bool fine_test(const Model& M, int o, const float4 q, unsigned& fl) {
-    if (o < M.n_ws) {
-        fl += kFlopsSS;
-        return MARGIN ? dist_sphere(q.x, q.y, q.z, q.w, M.wsph[o]) < M.margin : hit_sphere(q.x, q.y, q.z, q.w, M.wsph[o]);
+    if (o < M.n_wh) {
+        fl += kFlopsSH;
+        return MARGIN ? dist_halfspace(q.x, q.y, q.z, q.w, M.whs[o]) < M.margin
+                      : hit_halfspace(q.x, q.y, q.z, q.w, M.whs[o]);
     }
-    o -= M.n_ws;
+    o -= M.n_wh;
+    if (o < M.n_wb) {
+        fl += kFlopsSB;
+        return MARGIN ? dist_box(q.x, q.y, q.z, q.w, M.wbox[o]) < M.margin : hit_box(q.x, q.y, q.z, q.w, M.wbox[o]);
+    }
+    o -= M.n_wb;
     if (o < M.n_wc) {
         fl += kFlopsSC;
         return MARGIN ? dist_capsule(q.x, q.y, q.z, q.w, M.wcap[o]) < M.margin
                       : hit_capsule(q.x, q.y, q.z, q.w, M.wcap[o]);
     }
     o -= M.n_wc;
-    if (o < M.n_wb) {
-        fl += kFlopsSB;
-        return MARGIN ? dist_box(q.x, q.y, q.z, q.w, M.wbox[o]) < M.margin : hit_box(q.x, q.y, q.z, q.w, M.wbox[o]);
-    }
-    o -= M.n_wb;
-    fl += kFlopsSH;
-    return MARGIN ? dist_halfspace(q.x, q.y, q.z, q.w, M.whs[o]) < M.margin
-                  : hit_halfspace(q.x, q.y, q.z, q.w, M.whs[o]);
+    fl += kFlopsSS;
+    return MARGIN ? dist_sphere(q.x, q.y, q.z, q.w, M.wsph[o]) < M.margin : hit_sphere(q.x, q.y, q.z, q.w, M.wsph[o]);
 }
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
@@ -563,6 +565,9 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     const int no = nws + nwc + nwb + nwh;
     const long long total = static_cast<long long>(no) * ng;
     int qn = 0;
+    // Refine as soon as one pair is queued, then wait for twice as many after each clear
+    // refinement: a colliding configuration exits early, a free one refines in full batches.
+    int flush_at = 4;
     auto refine = [&]() -> bool {
         wave_sync();
         bool h = false;
@@ -591,13 +596,16 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
             entry = (o << 6) | g;
         }
         const long long p1 = min(p0 + kWave, total);
-        flops += pairs_in(p0, p1, ng, 0, nws) * kFlopsSS + pairs_in(p0, p1, ng, nws, nws + nwc) * kFlopsSC +
-                 pairs_in(p0, p1, ng, nws + nwc, nws + nwc + nwb) * kFlopsSB +
-                 pairs_in(p0, p1, ng, nws + nwc + nwb, no) * kFlopsSH;
+        flops += pairs_in(p0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(p0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
+                 pairs_in(p0, p1, ng, nwh + nwb, nwh + nwb + nwc) * kFlopsSC +
+                 pairs_in(p0, p1, ng, nwh + nwb + nwc, no) * kFlopsSS;
         const unsigned long long mask = __ballot(dirty);
         if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
         qn += static_cast<int>(__popcll(mask));
-        if (qn >= kWave && refine()) return done(true);
+        if (qn >= flush_at) {
+            if (refine()) return done(true);
+            flush_at = min(2 * flush_at, kWave);
+        }
     }
     if (qn > 0 && refine()) return done(true);
     for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
