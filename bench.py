@@ -314,12 +314,13 @@ def main():
     qcfg = torch.from_numpy(rng.uniform(sc2.robot.lower, sc2.robot.upper,
                                         size=(args.fkcc_configs, 7)).astype(np.float32)).cuda()
     fkcc = {}
-    # verdicts (binary; margin 0.02) use the coarse-culled check; the min-distance leg is the
+    # verdicts (binary; margin 0.005) use the coarse-culled check; the min-distance leg is the
     # flat full sweep over every sphere pair (no early exit, no culling)
     legs = (("binary", "binary", True), ("margin_verdict", "differentiable", True),
             ("min_distance_full_sweep", "differentiable", False))
     for label, mode, verdict in (legs if args.fkcc_configs > 0 else ()):
-        chk = CollisionChecker(sc2.context(), 7, collision_checker=mode, collision_margin=0.02)
+        # margin 0.005: the floor lies 0.01 m under the base spheres, so 0.02 flags every config
+        chk = CollisionChecker(sc2.context(), 7, collision_checker=mode, collision_margin=0.005)
         hit = torch.empty(args.fkcc_configs, dtype=torch.uint8, device="cuda") if verdict else None
         mind = None if verdict else torch.empty(args.fkcc_configs, dtype=torch.float32, device="cuda")
         flops = torch.zeros(1, dtype=torch.int64, device="cuda")
