@@ -132,7 +132,8 @@ size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nod
  * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] launch shape chosen (bits 0-15
  * resident work-groups, 16-31 threads per work-group, 32-47 pass-1 rounds, 48-63 team cap),
  * [4..9] / [10..15] device wall-clock ticks spent in the round phases of the per-query pass /
- * the team pass (sample+NN, extend check, commit, connect NN, connect walk, connect commit).
+ * the team pass (sample+NN, extend check with the connect NN, commit, unused, connect walk,
+ * connect commit).
  * No reference counterpart (the reference reports only kernel_time_ms). */
 size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
 

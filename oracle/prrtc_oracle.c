@@ -9,7 +9,8 @@
  *   cuda-rrtc/CUDA/prrtc_planner.cu:675-1066  (prrtc_planner_kernel)
  * under ONE fixed schedule of that racy kernel -- the lockstep schedule:
  *   every explorer (= reference block) runs phase k of round r before any explorer starts
- *   phase k+1, and shared-state updates are applied in explorer order.
+ *   phase k+1, and shared-state updates are applied in explorer order; an explorer's connect
+ *   nearest-neighbour search sees the other tree as it stood at the start of the round.
  * Every lockstep execution is one legal interleaving of the reference's blocks (they have no
  * ordering between them), so the lockstep planner is "the reference algorithm with the race
  * resolved".  The GPU library implements the same schedule; its trees must equal these
@@ -591,12 +592,14 @@ int oc_plan(const oc_context* c, const oc_params* p, const float* start, const f
             newi[e] = idx;
         }
         if (status != 0) break;
-        /* phase 4: connect nearest neighbour in the other tree */
+        /* phase 4: connect nearest neighbour in the other tree, as the tree stood at the start of
+         * the round (the interleaving in which every explorer's connect search runs before the
+         * other explorers' appends of this round; the GPU does it during the extend checks) */
         for (int e = 0; e < S; ++e) {
             if (newi[e] < 0) continue;
             const oc_tree* O = &tr[1 - side[e]];
             float bd;
-            nn_scan(O->cfg, N, O->size, dim, &cand[e * dim], &onn[e], &bd);
+            nn_scan(O->cfg, N, side[e] ? size_a : size_b, dim, &cand[e * dim], &onn[e], &bd);
             const float cd = sqrtf(bd);
             int n_ext = 0;
             if (p->step_size > 1e-8f && cd > 1e-8f) n_ext = (int)ceilf(cd / p->step_size);

@@ -8,8 +8,8 @@
 //   2  FK + collision of every extend edge (G waypoints each)  prrtc_planner.cu:883-909
 //   3  commit: all radius shrinks, then appends in explorer order with radius grows
 //                                                         prrtc_planner.cu:911-947, 46-83
-//   4  connect nearest neighbour in the other tree, step count and step vector
-//                                                         prrtc_planner.cu:949-994
+//   4  connect nearest neighbour in the other tree as it stood at the round's start (run
+//      alongside phase 2), step count and step vector      prrtc_planner.cu:949-994
 //   5  connect sub-edges, evaluated in passes until each explorer's first failing one is known
 //                                                         prrtc_planner.cu:996-1018
 //   6  connect appends in explorer order; the first success or overflow decides the query
@@ -359,7 +359,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     __syncthreads();
 
     unsigned long long evals = 0, flops = 0;  // lane 0 of each wave counts
-    // Per-phase wall time (member 0, thread 0): up to B1, B3, B4, B5, B7, B8.
+    // Per-phase wall time (member 0, thread 0): up to B1, B3 (extend checks and connect NN), the
+    // commit, (slot 3 unused), B7, B8.
     unsigned long long ph[6] = {0, 0, 0, 0, 0, 0};
     unsigned long long tph = wall_clock64();
     auto mark = [&](int k) {
@@ -475,6 +476,12 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         __syncthreads();
+        // ---- phase 4 (early): connect nearest neighbour of every candidate in the other tree as
+        // it stands at the start of the round -- the interleaving in which each explorer's
+        // connect search precedes the other explorers' appends of this round.  Computed for all
+        // candidates, so it needs no barrier after the commit; only successful ones use it.
+        nn_phase([&](int e) { return state[e] == 1 ? 1 - side[e] : -1; },
+                 [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, size_a, size_b);
         // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku
         // consecutive waypoints of one edge (FK one per lane, then checks in order; a collision
         // anywhere on the edge stops the remaining units of that edge).
@@ -605,14 +612,11 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 if (sc[SC_OVERFLOW]) sc[SC_STATUS] = PRRTC_STATUS_TREE_FULL;
             }
         }
-        team_barrier(team);  // B4: new nodes and radii are in global memory
+        // No team barrier: the connect walk reads only nodes that existed at the start of the round
+        // (appends and radius updates are seen from the next round, after B7 / B8).
+        __syncthreads();
         mark(2);
         if (sc[SC_STATUS] != 0) break;
-        // ---- phase 4: connect nearest neighbour in the other tree of each new node
-        nn_phase([&](int e) { return newi[e] >= 0 ? 1 - side[e] : -1; },
-                 [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, sc[SC_SIZE_A], sc[SC_SIZE_B]);
-        team_barrier(team);  // B5: connect nearest neighbours are in global memory
-        mark(3);
         for (int e = tid; e < S; e += nt) {
             if (newi[e] < 0) continue;
             const float* O = side[e] ? TA : TB;
