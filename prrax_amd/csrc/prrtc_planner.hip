@@ -476,12 +476,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         __syncthreads();
-        // ---- phase 4 (early): connect nearest neighbour of every candidate in the other tree as
-        // it stands at the start of the round -- the interleaving in which each explorer's
-        // connect search precedes the other explorers' appends of this round.  Computed for all
-        // candidates, so it needs no barrier after the commit; only successful ones use it.
-        nn_phase([&](int e) { return state[e] == 1 ? 1 - side[e] : -1; },
-                 [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, size_a, size_b);
+        // ---- phase 4 (early, teams): connect nearest neighbour of every candidate in the other
+        // tree as it stands at the start of the round -- the interleaving in which each
+        // explorer's connect search precedes the other explorers' appends of this round.  A team
+        // computes it for all candidates here, so no team barrier is needed after the commit; a
+        // single work-group searches after the commit, for the successful extends only.
+        if (W > 1)
+            nn_phase([&](int e) { return state[e] == 1 ? 1 - side[e] : -1; },
+                     [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, size_a, size_b);
         // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku
         // consecutive waypoints of one edge (FK one per lane, then checks in order; a collision
         // anywhere on the edge stops the remaining units of that edge).
@@ -617,6 +619,9 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         __syncthreads();
         mark(2);
         if (sc[SC_STATUS] != 0) break;
+        if (W == 1)
+            nn_phase([&](int e) { return newi[e] >= 0 ? 1 - side[e] : -1; },
+                     [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, size_a, size_b);
         for (int e = tid; e < S; e += nt) {
             if (newi[e] < 0) continue;
             const float* O = side[e] ? TA : TB;
