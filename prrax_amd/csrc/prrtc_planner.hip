@@ -404,6 +404,12 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             __syncthreads();
         }
     };
+    // Connect searches of this round's candidates (all = every candidate, else the successful
+    // extends) against the other tree's first sz0 / sz1 nodes.
+    auto connect_nn = [&](bool all, int sz0, int sz1) {
+        nn_phase([&](int e) { return (all ? state[e] == 1 : newi[e] >= 0) ? 1 - side[e] : -1; },
+                 [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, sz0, sz1);
+    };
     const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
     int r = r_begin;
     for (;; ++r) {
@@ -481,9 +487,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         // explorer's connect search precedes the other explorers' appends of this round.  A team
         // computes it for all candidates here, so no team barrier is needed after the commit; a
         // single work-group searches after the commit, for the successful extends only.
-        if (W > 1)
-            nn_phase([&](int e) { return state[e] == 1 ? 1 - side[e] : -1; },
-                     [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, size_a, size_b);
+        if (W > 1) connect_nn(true, size_a, size_b);
         // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku
         // consecutive waypoints of one edge (FK one per lane, then checks in order; a collision
         // anywhere on the edge stops the remaining units of that edge).
@@ -619,9 +623,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         __syncthreads();
         mark(2);
         if (sc[SC_STATUS] != 0) break;
-        if (W == 1)
-            nn_phase([&](int e) { return newi[e] >= 0 ? 1 - side[e] : -1; },
-                     [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, size_a, size_b);
+        if (W == 1) connect_nn(false, size_a, size_b);
         for (int e = tid; e < S; e += nt) {
             if (newi[e] < 0) continue;
             const float* O = side[e] ? TA : TB;
