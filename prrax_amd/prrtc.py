@@ -243,6 +243,15 @@ class PreparedBatch:
             ptr(self.order), self.attrs, out, launch, self.bufs.workspace.data_ptr(), self.bufs.workspace.numel())
         _lib.check("prrtc_planner_batch", rc)
 
+    def capture(self, stream: Optional[torch.cuda.Stream] = None) -> torch.cuda.CUDAGraph:
+        """Captures one launch (reset, pass 1, cooperative pass 2) into a HIP graph; replaying it
+        re-plans the batch with the same inputs.  The library clears its control words with a
+        kernel, not a memset, so replays are independent."""
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream if stream is not None else torch.cuda.Stream()):
+            self.enqueue()
+        return g
+
 
 def _limits(min_vals, max_vals, dim, device):
     lo = np.full(dim, -np.pi, np.float32) if min_vals is None else as_numpy(min_vals, np.float32).reshape(dim)

@@ -13,7 +13,8 @@ import torch
 
 from oracle import oracle as O
 from prrax_amd import scenes
-from prrax_amd.prrtc import CollisionChecker, forward_kinematics, prrtc_nearest_neighbor, prrtc_plan, prrtc_plan_batch, prrtc_extend
+from prrax_amd.prrtc import (CollisionChecker, forward_kinematics, prepare_batch, prrtc_extend,
+                             prrtc_nearest_neighbor, prrtc_plan, prrtc_plan_batch)
 
 from helpers import compare_plan, query
 
@@ -248,6 +249,42 @@ def test_dual_arm_batch_bit_exact():
         compare_plan(res[qi], ora, f"c4 q{qi}")
         compare_plan(res_t[qi], ora, f"c4 teams q{qi}")
         assert res[qi].solved
+
+
+def test_graph_replay_matches_eager():
+    """A batch captured in a HIP graph (reset kernel, pass 1, cooperative pass 2) replays to the
+    eager result, with eager launches and synchronisations between the replays."""
+    robot = scenes.make_arm(0, 64, 200)
+    ctxs, starts, goals, nodes = [], [], [], []
+    for qi in (0, 1, 2, 3, 26):
+        w, ng, N = scenes.c3_query_world(robot, qi)
+        cc = dict(robot.ctx)
+        cc.update(w)
+        s, g = query(scenes.Scene(robot, w), 3, qi, n_goals=ng)
+        ctxs.append(cc); starts.append(s); goals.append(g); nodes.append(N)
+    pb = prepare_batch(np.stack(starts), goals, collision_context=ctxs, max_nodes=max(nodes), max_nodes_per_query=nodes,
+                       num_new_samples=64, max_iterations=300, min_vals=robot.lower, max_vals=robot.upper,
+                       pass1_rounds=2)
+    say("graph capture")
+    pb.enqueue()
+    torch.cuda.synchronize()
+    ref = pb.bufs.results()
+    ref_trees = [(r.tree_a_configs.cpu().numpy().copy(), r.tree_b_parents.cpu().numpy().copy()) for r in ref]
+    g = pb.capture()
+    for k in range(3):
+        pb.bufs.ints.zero_()
+        pb.bufs.tree_a.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = pb.bufs.results()
+        for a, b, (ta, pbr) in zip(got, ref, ref_trees):
+            assert a.status == b.status and a.iterations == b.iterations
+            assert (a.tree_a_size, a.tree_b_size) == (b.tree_a_size, b.tree_b_size)
+            np.testing.assert_array_equal(a.tree_a_configs.cpu().numpy(), ta)
+            np.testing.assert_array_equal(a.tree_b_parents.cpu().numpy(), pbr)
+        pb.enqueue()  # eager work between replays
+        torch.cuda.synchronize()
 
 
 def test_paths_valid_under_fp64_checker():
