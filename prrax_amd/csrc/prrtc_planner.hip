@@ -10,7 +10,8 @@
 //                                                         prrtc_planner.cu:911-947, 46-83
 //   4  connect nearest neighbour in the other tree as it stood at the round's start (run
 //      alongside phase 2), step count and step vector      prrtc_planner.cu:949-994
-//   5  connect sub-edges, evaluated in passes until each explorer's first failing one is known
+//   5  connect sub-edges, checked in parallel work units; each explorer's first failing sub-edge
+//      is an order-independent minimum
 //                                                         prrtc_planner.cu:996-1018
 //   6  connect appends in explorer order; the first success or overflow decides the query
 //                                                         prrtc_planner.cu:1020-1062
