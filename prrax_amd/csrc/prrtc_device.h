@@ -454,6 +454,10 @@ __host__ __device__ inline int cull_stride_bytes(int nrs, int nj) {
 
 // Obstacles of the culled check are numbered halfspaces, boxes, capsules, spheres: the few large
 // primitives (a floor, walls) come first, where a colliding configuration is found soonest.
+// Distances are compared squared (d > L  <=>  d^2 > L^2 for L >= 0): no square roots, and the
+// rounding difference is far inside kCullPad.
+__device__ __forceinline__ bool beyond(float d2, float L) { return L < 0.0f || d2 > L * L; }
+
 __device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4 c, float lim) {
     if (o < M.n_wh) {
         const float4 h = M.whs[o];
@@ -468,17 +472,17 @@ __device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4
         float q0, q1, q2;
         box_q(c.x, c.y, c.z, b, q0, q1, q2);
         const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
-        return __builtin_sqrtf(__builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0))) > lim;
+        return beyond(__builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0)), lim);
     }
     o -= M.n_wb;
     if (o < M.n_wc) {
         const CapsuleL& k = M.wcap[o];
-        return __builtin_sqrtf(capsule_d2(c.x, c.y, c.z, k)) > lim + fabsf(k.r);
+        return beyond(capsule_d2(c.x, c.y, c.z, k), lim + fabsf(k.r));
     }
     o -= M.n_wc;
     const float4 w = M.wsph[o];
     const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
-    return __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) > lim + fabsf(w.w);
+    return beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), lim + fabsf(w.w));
 }
 
 // Fine test of robot sphere q (world x, y, z, r) against obstacle o: hit (binary) or distance
