@@ -50,7 +50,7 @@ __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, in
     L.joints = o; o = align16(o + sizeof(JointRec) * nj);
     L.spheres = o; o = align16(o + sizeof(float4) * nrs);
     L.sphere_js = o; o = align16(o + sizeof(int) * nrs);
-    L.pairs = o; o = align16(o + sizeof(int2) * nsp);
+    L.pairs = o; o = align16(o + sizeof(unsigned) * nsp);
     L.wsph = o; o = align16(o + sizeof(float4) * nws);
     L.wcap = o; o = align16(o + sizeof(CapsuleL) * nwc);
     L.wbox = o; o = align16(o + sizeof(BoxL) * nwb);
@@ -161,7 +161,7 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
     JointRec* J = reinterpret_cast<JointRec*>(lds + Lo.joints);
     float4* SP = reinterpret_cast<float4*>(lds + Lo.spheres);
     int* SJ = reinterpret_cast<int*>(lds + Lo.sphere_js);
-    int2* PR = reinterpret_cast<int2*>(lds + Lo.pairs);
+    unsigned* PR = reinterpret_cast<unsigned*>(lds + Lo.pairs);
     float4* WS = reinterpret_cast<float4*>(lds + Lo.wsph);
     CapsuleL* WC = reinterpret_cast<CapsuleL*>(lds + Lo.wcap);
     BoxL* WB = reinterpret_cast<BoxL*>(lds + Lo.wbox);
@@ -237,7 +237,7 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
         }
         int tot;
         const int pre = block_excl_scan(ok, scan_scratch, &tot);
-        if (ok) PR[base + pre] = make_int2(i, j);
+        if (ok) PR[base + pre] = static_cast<unsigned>(i) | (static_cast<unsigned>(j) << 16);
         base += tot;
     }
     // Culling groups: the robot spheres of each frame (frames in order), with a bounding sphere

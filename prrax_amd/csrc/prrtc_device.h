@@ -136,7 +136,7 @@ struct Model {
     const JointRec* joints;
     const float4* spheres;   // robot spheres: local xyz, radius
     const int* sphere_js;    // sorted-joint index of each robot sphere, -1 if skipped
-    const int2* pairs;       // valid self pairs
+    const unsigned* pairs;   // valid self pairs, i | j << 16 (robot spheres < 65536)
     const float4* wsph;
     const CapsuleL* wcap;
     const BoxL* wbox;
@@ -412,13 +412,13 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
         const int k = p0 + lane;
         bool h = false;
         if (k < M.n_pairs) {
-            const int2 pr = M.pairs[k];
-            const float4 a = M.spheres[pr.x];
-            const float4 b = M.spheres[pr.y];
+            const int pi = static_cast<int>(M.pairs[k] & 0xffffu), pj = static_cast<int>(M.pairs[k] >> 16);
+            const float4 a = M.spheres[pi];
+            const float4 b = M.spheres[pj];
             Mat3 Ri, Rj;
             float ix, iy, iz, jx, jy, jz;
-            frame_from(T, ts, M.sphere_js[pr.x], Ri, ix, iy, iz);
-            frame_from(T, ts, M.sphere_js[pr.y], Rj, jx, jy, jz);
+            frame_from(T, ts, M.sphere_js[pi], Ri, ix, iy, iz);
+            frame_from(T, ts, M.sphere_js[pj], Rj, jx, jy, jz);
             const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
             const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
             if (MARGIN) best = fminf(best, dist_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w)));
@@ -616,8 +616,7 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         const int k = p0 + lane;
         bool h = false;
         if (k < M.n_pairs) {
-            const int2 pr = M.pairs[k];
-            const float4 a = FP[pr.x], b = FP[pr.y];
+            const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
             h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
         }
         flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;

@@ -56,7 +56,8 @@ int check_robot(const prrtc_robot* r) {
     if (!r) return fail_with(PRRTC_E_INVALID, "robot is null");
     if (r->n_joints < 0 || r->n_joints > PRRTC_MAX_JOINTS)
         return fail_with(PRRTC_E_INVALID, "n_joints must be in [0, 64]");
-    if (r->n_robot_spheres < 0) return fail_with(PRRTC_E_INVALID, "n_robot_spheres < 0");
+    if (r->n_robot_spheres < 0 || r->n_robot_spheres > 65535)
+        return fail_with(PRRTC_E_INVALID, "n_robot_spheres must be in [0, 65535]");
     if (r->n_joints > 0 && (!r->fk_twists || !r->fk_parent_tf || !r->fk_parent_idx || !r->fk_act_idx ||
                             !r->fk_mimic_mul || !r->fk_mimic_off || !r->fk_mimic_act_idx || !r->fk_topo_inv))
         return fail_with(PRRTC_E_INVALID, "FK arrays must be non-null when n_joints > 0");

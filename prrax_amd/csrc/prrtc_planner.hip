@@ -217,11 +217,11 @@ __device__ __forceinline__ unsigned long long nn_key(float dsq, int idx) {
 }
 constexpr unsigned long long kNoKey = ~0ull;
 
-// Waypoints per work unit of a check phase holding `work` waypoints: kFB (checks in order, so a
+// Waypoints per work unit of a check phase holding `work` waypoints: fb (checks in order, so a
 // collision skips the rest of the unit) unless the team has more waves than units, in which case
 // units shrink (down to one waypoint) so that every wave gets one.  Uniform over the team.
-__device__ __forceinline__ int unit_size(long long work, int waves) {
-    int ku = kFB;
+__device__ __forceinline__ int unit_size(long long work, int waves, int fb) {
+    int ku = fb;
     while (ku > 1 && work / ku < waves) ku >>= 1;
     return ku;
 }
@@ -334,10 +334,11 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     Model M;
     stage_model(M, lds, a.ml, a.robot, a.world, V.q, a.dim, a.mode, a.margin, scan);
     const WaveArea WA = wave_area(lds, a.ml, M.n_joints);
-    LaneScratch L;  // FK slot of this lane (lanes < kFB) in the wave's FK area
+    const int fbp = a.ml.fbp, fb = fbp - 1;  // FK batch: fb configurations per wave, stride fbp
+    LaneScratch L;  // FK slot of this lane (lanes < fb) in the wave's FK area
     L.q = WA.q + lane;
     L.T = WA.T + lane;
-    L.nt = kFBP;
+    L.nt = fbp;
     float* TA = V.TA;
     float* TB = V.TB;
     float* RA = V.RA;
@@ -494,7 +495,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         // anywhere on the edge stops the remaining units of that edge).
         {
             const int nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
-            const int ku = unit_size(static_cast<long long>(nl) * G, W * nwaves);
+            const int ku = unit_size(static_cast<long long>(nl) * G, W * nwaves, fb);
             const int cps = (G + ku - 1) / ku;
             // Fewer waypoints than waves: a fixed round-robin deal of one-waypoint units (no
             // claiming; nothing is left to skip).  Otherwise units are claimed dynamically (claim
@@ -539,7 +540,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
                     for (int d = 0; d < dim; ++d) {
                         const float base = ldc<COH>(T + static_cast<size_t>(d) * ncap + nn[e]);
-                        L.q[d * kFBP] = base + tf * (cand[e * dim + d] - base);
+                        L.q[d * fbp] = base + tf * (cand[e * dim + d] - base);
                     }
                     if (M.enabled) fk_lane(M, L);
                 }
@@ -547,7 +548,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 for (int c = 0; c < nk; ++c) {
                     if (uniform(gload(&V.g_fail[e]))) break;
                     if (lane == 0) ++evals;
-                    if (wave_in_collision(M, WA.T + c, kFBP, flops)) {
+                    if (wave_in_collision(M, WA.T + c, fbp, flops)) {
                         if (lane == 0) gstore(&V.g_fail[e], 1);
                         break;
                     }
@@ -653,7 +654,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 block_excl_scan(k < nl ? next[list[k]] : 0, scan, &tot);
                 nsub += tot;
             }
-            const int ku = unit_size(static_cast<long long>(nsub) * G, W * nwaves);
+            const int ku = unit_size(static_cast<long long>(nsub) * G, W * nwaves, fb);
             const int cps = (G + ku - 1) / ku;
             // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers in
             // aux0).  Otherwise units are claimed dynamically, and a collision at sub-edge j lowers
@@ -722,7 +723,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                         float c = cand[e * dim + d];
                         const float v = vec[e * dim + d];
                         for (int q = 0; q < jj; ++q) c += v;
-                        L.q[d * kFBP] = c + tf * v;
+                        L.q[d * fbp] = c + tf * v;
                     }
                     if (M.enabled) fk_lane(M, L);
                 }
@@ -730,7 +731,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 for (int c = 0; c < nk; ++c) {
                     if (jj >= uniform(gload(&V.g_cfail[e]))) break;
                     if (lane == 0) ++evals;
-                    if (wave_in_collision(M, WA.T + c, kFBP, flops)) {
+                    if (wave_in_collision(M, WA.T + c, fbp, flops)) {
                         if (lane == 0) gmin(&V.g_cfail[e], jj);
                         break;
                     }
@@ -1142,19 +1143,33 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     a.stats = reinterpret_cast<unsigned long long*>(ws + wl.stats);
     a.wall_khz = di.wall_khz;
 
+    // LDS: the FK batch (16 configurations per wave, or 8 when that lets more work-groups stay
+    // resident, as for large robots), shrinking the work-group if even that does not fit.
+    const bool small = dim <= 8;
     size_t lds = 0;
+    int per_cu = 0;
     for (;;) {
-        a.ml = model_layout(robot->n_joints, robot->n_robot_spheres, world->n_self_pairs, world->n_world_spheres,
-                            world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt / kWave, kFBP);
-        lds = planner_lds(a.ml, S, dim, a);
-        if (lds <= static_cast<size_t>(di.max_lds) || nt <= kWave) break;
+        for (const int fbp : {kFBP, kFB / 2 + 1}) {
+            PlanArgs t = a;
+            t.ml = model_layout(robot->n_joints, robot->n_robot_spheres, world->n_self_pairs, world->n_world_spheres,
+                                world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt / kWave,
+                                fbp);
+            const size_t l = planner_lds(t.ml, S, dim, t);
+            if (l > static_cast<size_t>(di.max_lds)) continue;
+            const int occ = small ? occupancy<8>(nt, l) : occupancy<16>(nt, l);
+            if (occ > per_cu) {
+                per_cu = occ;
+                lds = l;
+                a = t;
+            }
+        }
+        if (per_cu > 0 || nt <= kWave) break;
         nt -= kWave;
     }
-    if (lds > static_cast<size_t>(di.max_lds))
-        return fail_with(PRRTC_E_LDS, "robot + world + explorer state exceed the work-group LDS (" +
-                                           std::to_string(lds) + " > " + std::to_string(di.max_lds) + " bytes)");
-    const bool small = dim <= 8;
-    const int resident = (small ? occupancy<8>(nt, lds) : occupancy<16>(nt, lds)) * di.cus;
+    if (per_cu == 0)
+        return fail_with(PRRTC_E_LDS, "robot + world + explorer state exceed the work-group LDS (max " +
+                                           std::to_string(di.max_lds) + " bytes)");
+    const int resident = per_cu * di.cus;
     const int slots = (launch && launch->slots > 0) ? std::min(launch->slots, resident) : resident;
     a.slots = std::max(1, std::min(slots, kMaxSlots));
     // A lone query goes straight to a team; a batch first gives every query a short solo run.
