@@ -54,15 +54,15 @@ def tuning_sample(contexts: Sequence[dict], n: int = 96, n_heavy: int = 16) -> l
 
 def autotune(starts, goals, collision_context, *, max_nodes_per_query=None, samples: Sequence[int] = (128,),
              threads: Sequence[int] = (128, 256), pass1_rounds: Sequence[int] = (24,),
-             team_max: Sequence[int] = (256, 512), n_sample: int = 96, repeats: int = 1,
+             team_max: Sequence[int] = (0,), n_sample: int = 96, repeats: int = 1,
              log=None, **plan_kw) -> TuneResult:
     """Times every candidate on a tuning sample of the batch and returns the fastest whose solve
     rate is at least the default S's.  ``plan_kw`` are the planner keywords of the real run
     (max_iterations, step_size, limits, ...); ``collision_context`` is one dict or one per query.
 
-    By default only the launch shape is tuned (S stays 128): a sample rarely holds the few
-    queries that never connect, and those set a batch's makespan, so an S chosen on the sample
-    can lose on the whole batch.  ``pass1_rounds`` is pinned (24, the full-batch default) so the
+    By default only the work-group size is tuned (S stays 128, the team cap stays the library's):
+    a sample rarely holds the few queries that never connect, and those set a batch's makespan,
+    so an S or a team cap chosen on the sample can lose on the whole batch.  ``pass1_rounds`` is pinned (24, the full-batch default) so the
     sample runs in the same regime as a batch larger than the resident work-groups."""
     B = len(starts)
     ctxs = collision_context if isinstance(collision_context, (list, tuple)) else [collision_context] * B
