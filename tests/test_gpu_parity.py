@@ -166,6 +166,22 @@ def test_planner_variants_bit_exact(kw):
     compare_plan(res, ora, str(kw))
 
 
+@pytest.mark.parametrize("S,G,N,team", [(1, 1, 64, 0), (1, 16, 4096, 8), (3, 2, 200, 512), (130, 5, 3000, 2),
+                                         (257, 16, 300, 64)])
+def test_planner_edge_shapes_bit_exact(S, G, N, team):
+    """Odd explorer counts (1, 3, one past a wave, one past a work-group), granularity 1/2/5,
+    tiny trees and several team shapes, against the oracle."""
+    sc = scenes.scene_c1()
+    cc = sc.context()
+    start, goals = query(sc, 1, 30, n_goals=2)
+    kw = dict(num_new_samples=S, granularity=G, max_nodes=N, max_iterations=150, min_vals=sc.robot.lower,
+              max_vals=sc.robot.upper)
+    say(f"edge shape S={S} G={G} N={N} team={team}")
+    res = prrtc_plan_batch(start[None], [goals], collision_context=cc, team_max=team, pass1_rounds=0, **kw)[0]
+    ora = O.plan(O.Context(cc, 7), start, goals, **kw)
+    compare_plan(res, ora, f"S{S} G{G} N{N}")
+
+
 def test_planner_failure_modes_bit_exact():
     sc = scenes.scene_c1()
     cc = sc.context()
