@@ -90,6 +90,35 @@ def copy_calibration(mib: int):
     log(f"copy calibration: {mib} MiB read + {mib} MiB written")
 
 
+def scale_leg(n_queries: int, checker_factory, kw: dict, launch: dict):
+    """The same uneven-query generator at a larger batch, one launch: the few queries that never
+    connect bound a 1024-query step, and a larger batch spreads them over more plans."""
+    import torch
+
+    from prrax_amd import prepare_batch
+
+    t0 = time.perf_counter()
+    robot, worlds, starts, goals, nodes = build_c3(n_queries, checker_factory)
+    log(f"scaling: {n_queries} queries built in {time.perf_counter() - t0:.1f}s")
+    order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
+                       kind="stable")
+    pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
+                       max_nodes_per_query=nodes, order=order, **launch, **kw)
+    pb.enqueue()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pb.enqueue()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    h = pb.bufs.host_summary()
+    out = {"queries": n_queries, "ms": round(dt * 1000, 1), "plans_per_s": round(n_queries / dt, 1),
+           "solve_rate": round(float(np.mean(h["solved"] == 1)), 4),
+           "longest_query_ms": round(float(h["kernel_time_ms"].max()), 1)}
+    log(f"scaling: {out}")
+    del pb
+    return out
+
+
 def dual_arm_leg(n_queries: int):
     """14-DOF dual arm with mimic grippers (128 spheres, ~2000 self pairs), 150 spheres + 12
     boxes + floor, goals 1-4 per query, step 0.3: one batched launch vs per-query launches."""
@@ -216,6 +245,8 @@ def main():
     ap.add_argument("--team-max", type=int, default=0, help="largest work-group team per open query (0 = library default)")
     ap.add_argument("--refine-paths", type=int, default=512, help="paths of the refinement leg (0 = skip)")
     ap.add_argument("--dual-arm-queries", type=int, default=256, help="queries of the 14-DOF leg (0 = skip)")
+    ap.add_argument("--scale-queries", type=int, default=4096,
+                    help="queries of the batch-size scaling leg (same generator, one launch; 0 = skip)")
     ap.add_argument("--no-autotune", action="store_true", help="use S = --samples and the library's launch defaults")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -342,6 +373,7 @@ def main():
         if verdict:
             fkcc[label]["in_collision"] = round(float(hit.float().mean().item()), 4)
         log(f"fkcc {label}: {fkcc[label]}")
+    scale = scale_leg(args.scale_queries, checker_factory, kw, launch) if args.scale_queries > 0 else None
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
@@ -376,6 +408,7 @@ def main():
         "autotune": None if tuned is None else {
             "chosen": tuned.kwargs(), "sample_plans_per_s": round(tuned.plans_per_s, 1),
             "candidates": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()} for r in tuned.table]},
+        "batch_scaling": scale,
         "refine": refine,
         "dual_arm": dual,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
