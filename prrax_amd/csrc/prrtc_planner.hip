@@ -1028,10 +1028,21 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
                 for (int L = wtop; L >= 1; L >>= 1)
                     for (int k = 0; k < ncont; ++k)
                         if (TW[k] == L) { TO[k] = off; off += L; }
-                const int v = (a.slots % 8 == 0) ? (b % 8) * (a.slots / 8) + b / 8 : b;
+                // v numbers the work-groups XCD by XCD (they are dealt to XCDs round-robin); a team
+                // within one XCD keeps that order, a team over several XCDs numbers its members
+                // across the XCDs, so the low member numbers that get the first work units are
+                // spread over all of them.
+                const int per = a.slots / 8;
+                const bool by_xcd = a.slots % 8 == 0;
+                const int v = by_xcd ? (b % 8) * per + b / 8 : b;
                 mine[0] = -1;
-                for (int k = 0; k < ncont; ++k)
-                    if (v >= TO[k] && v < TO[k] + TW[k]) { mine[0] = k; mine[1] = v - TO[k]; mine[2] = TW[k]; }
+                for (int k = 0; k < ncont; ++k) {
+                    if (v < TO[k] || v >= TO[k] + TW[k]) continue;
+                    const int loc = v - TO[k];
+                    mine[0] = k;
+                    mine[1] = (by_xcd && TW[k] > per) ? (loc % per) * (TW[k] / per) + loc / per : loc;
+                    mine[2] = TW[k];
+                }
             }
             __syncthreads();
             const int k = mine[0], member = mine[1], W = mine[2];
