@@ -965,8 +965,6 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
     }
 }
 
-constexpr int kPropMax = 256;  // open queries up to which pass 2 sizes teams by predicted work
-
 // Pass 2: teams of W work-groups resume the queries left open by pass 1, in epochs.  In each
 // epoch the open queries are dealt to teams (W = slots / open queries, a power of two, capped)
 // and run for at most 4 << epoch rounds (capped at `epoch_rounds`); the queries still open then
@@ -986,83 +984,6 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
         if (ncont == 0) return;  // every work-group reads the same count after the grid barrier
         // list (ep + 2) % 3 was read in the previous epoch and is filled in the next one
         if (b == 0 && threadIdx.x == 0) gstore(&a.ctl[CTL_NC + (ep + 2) % 3], 0);
-        // Epoch budgets grow 4, 8, 16, ... up to epoch_rounds: early epochs end soon, so the
-        // teams re-form (and grow) while most queries of a batch finish.
-        const int budget = (a.epoch_rounds > 0) ? min(a.epoch_rounds, 4 << min(ep, 20)) : -1;
-        if (2 * ncont <= a.slots && ncont <= kPropMax) {
-            // Few open queries: team sizes in proportion to each query's predicted remaining rounds
-            // (its remaining tree capacity over its growth so far), powers of two packed buddy-style
-            // into work-group ranges that follow the XCDs, so the likely longest query gets the
-            // largest team.  Every work-group computes the same plan from the same saved state.
-            float* P = reinterpret_cast<float*>(lds);
-            int* TW = reinterpret_cast<int*>(P + kPropMax);
-            int* TO = TW + kPropMax;
-            int* mine = TO + kPropMax;
-            for (int k = threadIdx.x; k < ncont; k += blockDim.x) {
-                const QueryView V = query_view(a, a.cont[cur * a.B + k]);
-                const int big = max(V.qs[QS_SIZE_A], V.qs[QS_SIZE_B]);
-                const float done = static_cast<float>(max(V.qs[QS_ROUND] - 1, 1));
-                const float grown = static_cast<float>(max(big - 1, 1));
-                P[k] = fmaxf(1.0f, static_cast<float>(max(V.N - big, 1)) * done / grown);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                float tot = 0.0f;
-                for (int k = 0; k < ncont; ++k) tot += P[k];
-                int sum = 0, wtop = 1;
-                for (int k = 0; k < ncont; ++k) {
-                    const float share = static_cast<float>(a.slots) * P[k] / tot;
-                    int w = 1;
-                    while (w * 2 <= a.wmax && static_cast<float>(w * 2) <= share) w *= 2;
-                    TW[k] = w;
-                    sum += w;
-                    wtop = max(wtop, w);
-                }
-                while (sum > a.slots) {  // shares below one work-group were rounded up: shrink the largest
-                    int kb = 0;
-                    for (int k = 1; k < ncont; ++k) if (TW[k] > TW[kb]) kb = k;
-                    TW[kb] >>= 1;
-                    sum -= TW[kb];
-                }
-                int off = 0;
-                for (int L = wtop; L >= 1; L >>= 1)
-                    for (int k = 0; k < ncont; ++k)
-                        if (TW[k] == L) { TO[k] = off; off += L; }
-                // v numbers the work-groups XCD by XCD (they are dealt to XCDs round-robin); a team
-                // within one XCD keeps that order, a team over several XCDs numbers its members
-                // across the XCDs, so the low member numbers that get the first work units are
-                // spread over all of them.
-                const int per = a.slots / 8;
-                const bool by_xcd = a.slots % 8 == 0;
-                const int v = by_xcd ? (b % 8) * per + b / 8 : b;
-                mine[0] = -1;
-                for (int k = 0; k < ncont; ++k) {
-                    if (v < TO[k] || v >= TO[k] + TW[k]) continue;
-                    const int loc = v - TO[k];
-                    mine[0] = k;
-                    mine[1] = (by_xcd && TW[k] > per) ? (loc % per) * (TW[k] / per) + loc / per : loc;
-                    mine[2] = TW[k];
-                }
-            }
-            __syncthreads();
-            const int k = mine[0], member = mine[1], W = mine[2];
-            __syncthreads();
-            if (k >= 0) {
-                unsigned* ts = a.tsync + 2 * k;
-                Team team{W, member, ts, a.ctl + CTL_ERROR, 0u};
-                if (threadIdx.x == 0)  // the pair is at rest: every barrier on it ended before the grid barrier
-                    team.gen = __hip_atomic_load((PRRTC_GLOBAL unsigned*)(ts + 1), __ATOMIC_ACQUIRE,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long t0 = wall_clock64();
-                const QueryView V = query_view(a, a.cont[cur * a.B + k]);
-                run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], budget, t0, 2, a.ctl + CTL_NC + nxt,
-                                      a.cont + nxt * a.B);
-                team_barrier(team);
-            }
-            grid_barrier(grid);
-            continue;
-        }
-        // Many open queries: equal teams of W (a power of two), queries dealt round-robin.
         int W = 1;
         while (W * 2 <= a.wmax && W * 2 * ncont <= a.slots) W *= 2;
         int team_id, member;
@@ -1074,6 +995,9 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
             member = b % W;
         }
         const int nteams = a.slots / W;
+        // Epoch budgets grow 4, 8, 16, ... up to epoch_rounds: early epochs end soon, so the
+        // teams re-form (and grow) while most queries of a batch finish.
+        const int budget = (a.epoch_rounds > 0) ? min(a.epoch_rounds, 4 << min(ep, 20)) : -1;
         if (team_id < nteams && team_id < ncont) {
             unsigned* ts = a.tsync + 2 * team_id;
             Team team{W, member, ts, a.ctl + CTL_ERROR, 0u};
@@ -1084,7 +1008,7 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
                 const unsigned long long t0 = wall_clock64();
                 const QueryView V = query_view(a, a.cont[cur * a.B + k]);
                 run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], budget, t0, 2,
-                                      a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
+                                a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
                 team_barrier(team);  // members leave the query together before the next one
             }
         }
@@ -1130,7 +1054,7 @@ size_t planner_lds(const ModelLayout& ml, int S, int dim, PlanArgs& a) {
     a.hash_size = hash_size_for(S);
     a.off_hash = o; o = align16(o + 3 * sizeof(int) * static_cast<size_t>(a.hash_size));
     a.off_misc = o; o = align16(o + sizeof(int) * (20 + SC_COUNT));
-    return std::max(o, sizeof(float) * 3 * kPropMax + 16);  // pass 2's team plan reuses the LDS
+    return o;
 }
 
 template <int MAXD>
