@@ -106,12 +106,12 @@ __device__ __forceinline__ void global_add(unsigned long long* p, unsigned long 
 template <bool COH, class T>
 __device__ __forceinline__ T ldc(const T* p) {
     if constexpr (COH) return __hip_atomic_load((const PRRTC_GLOBAL T*)(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else return *p;
+    else return *(const PRRTC_GLOBAL T*)(p);
 }
 template <bool COH, class T>
 __device__ __forceinline__ void stc(T* p, T v) {
     if constexpr (COH) __hip_atomic_store((PRRTC_GLOBAL T*)(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else *p = v;
+    else *(PRRTC_GLOBAL T*)(p) = v;
 }
 
 // ---------------------------------------------------------------------------------- scans
@@ -316,43 +316,66 @@ __device__ __forceinline__ bool lex_less(float da, int ia, float db, int ib) {
 // Nearest node (lowest index on ties) of kEC queries (qsh[c * kMaxDim + d], unused rows zero)
 // among tree[begin, end).  Every thread of the work-group calls it; on return red_d[c * 16] and
 // red_i[c * 16] hold query c's squared distance and index ((FLT_MAX, 0) for an empty range).
+// The buffers are passed as LDS pointers (cast by the caller, see lds_f / lds_i): this function
+// is not inlined, and generic pointers would make every query read a flat load that waits for
+// all outstanding memory traffic.
+__device__ __forceinline__ PRRTC_LDS float* lds_f(float* p) { return (PRRTC_LDS float*)(p); }
+__device__ __forceinline__ PRRTC_LDS int* lds_i(int* p) { return (PRRTC_LDS int*)(p); }
 template <int MAXD, bool COH = false>
-__device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int end, int dim, const float* qsh,
-                        float* red_d, int* red_i) {
+__device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int end, int dim,
+                        const PRRTC_LDS float* qsh, PRRTC_LDS float* red_d, PRRTC_LDS int* red_i) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int nw = (nt + kWave - 1) / kWave;
     float bd[kEC];
     int bi[kEC];
 #pragma unroll
     for (int c = 0; c < kEC; ++c) { bd[c] = FLT_MAX; bi[c] = 0; }
+    // Dimensions d >= dim are zero in the node (x) and in every query row, and fma(0, 0, s) == s,
+    // so summing all MAXD terms gives the dim-term sum exactly, without a branch per term.  The
+    // query rows are re-read (LDS broadcast) for every node: qoff is an opaque zero that
+    // keeps the compiler from hoisting all kEC * MAXD query values into registers.
+    int qoff = 0;
     for (int i = begin + tid; i < end; i += nt) {
+        asm volatile("" : "+v"(qoff));
         float x[MAXD];
 #pragma unroll
-        for (int d = 0; d < MAXD; ++d) x[d] = (d < dim) ? ldc<COH>(tree + static_cast<size_t>(d) * ncap + i) : 0.0f;
+        for (int d = 0; d < MAXD; ++d) {
+            const float v = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + i);
+            x[d] = (d < dim) ? v : 0.0f;
+        }
 #pragma unroll
         for (int c = 0; c < kEC; ++c) {
+            const PRRTC_LDS float* qr = qsh + c * kMaxDim + qoff;
             float dsq = 0.0f;
 #pragma unroll
             for (int d = 0; d < MAXD; ++d) {
-                if (d < dim) {
-                    const float df = x[d] - qsh[c * kMaxDim + d];
-                    dsq = __builtin_fmaf(df, df, dsq);
-                }
+                const float df = x[d] - qr[d];
+                dsq = __builtin_fmaf(df, df, dsq);
             }
             if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
         }
     }
+    // Minimum over the work-group of the keys (squared distance bits, index): distances are
+    // non-negative, so the key order is lex_less's.  Level-major, so that the kEC shuffle chains
+    // of a level are independent and overlap; the minimum is a select, not a branch.
+    unsigned long long key[kEC];
 #pragma unroll
-    for (int c = 0; c < kEC; ++c) {
-        float d = bd[c];
-        int i = bi[c];
+    for (int c = 0; c < kEC; ++c)
+        key[c] = (static_cast<unsigned long long>(__float_as_uint(bd[c])) << 32) | static_cast<unsigned>(bi[c]);
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const float od = __shfl_xor(d, o, kWave);
-            const int oi = __shfl_xor(i, o, kWave);
-            if (lex_less(od, oi, d, i)) { d = od; i = oi; }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+#pragma unroll
+        for (int c = 0; c < kEC; ++c) {
+            const unsigned long long ok = __shfl_xor(key[c], o, kWave);
+            key[c] = (ok < key[c]) ? ok : key[c];
         }
-        if (lane == 0) { red_d[c * 16 + wid] = d; red_i[c * 16 + wid] = i; }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < kEC; ++c) {
+            red_d[c * 16 + wid] = __uint_as_float(static_cast<unsigned>(key[c] >> 32));
+            red_i[c * 16 + wid] = static_cast<int>(key[c] & 0xffffffffull);
+        }
     }
     __syncthreads();
     if (tid < kEC) {
@@ -385,7 +408,7 @@ __device__ void nn_tiles(const float* __restrict__ tree, int ncap, int size, int
             qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qbuf[list[c0 + c] * dim + d] : 0.0f;
         }
         __syncthreads();
-        nn_tile<MAXD>(tree, ncap, 0, size, dim, qsh, red_d, red_i);
+        nn_tile<MAXD>(tree, ncap, 0, size, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i));
         if (tid < nc) {
             const int e = list[c0 + tid];
             out_i[e] = red_i[tid * 16];

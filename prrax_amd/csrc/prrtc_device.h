@@ -567,7 +567,13 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     wave_sync();
     const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
     const int no = nws + nwc + nwb + nwh;
-    const long long total = static_cast<long long>(no) * ng;
+    const int total = no * ng;  // world primitives are staged in LDS: far below 2^31 / 64
+    // Flops of the coarse pairs [0, p1) (counted once, where the scan stops).
+    auto coarse_flops = [&](int p1) {
+        return pairs_in(0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
+               pairs_in(0, p1, ng, nwh + nwb, nwh + nwb + nwc) * kFlopsSC +
+               pairs_in(0, p1, ng, nwh + nwb + nwc, no) * kFlopsSS;
+    };
     int qn = 0;
     // Refine as soon as one pair is queued, then wait for twice as many after each clear
     // refinement: a colliding configuration exits early, a free one refines in full batches.
@@ -589,27 +595,32 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         wave_sync();
         return any;
     };
-    for (long long p0 = 0; p0 < total; p0 += kWave) {
-        const long long p = p0 + lane;
-        bool dirty = false;
-        int entry = 0;
-        if (p < total) {
-            const int o = static_cast<int>(p / ng), g = static_cast<int>(p - static_cast<long long>(o) * ng);
-            const float4 c = CW[g];
-            dirty = !coarse_clear(M, o, c, c.w + cut + kCullPad);
-            entry = (o << 6) | g;
+    // Pair p = o * ng + g (obstacle-major); lane l takes p0 + l, and (o, g) advance by kWave
+    // pairs per step without a division (ng <= 64).
+    if (total > 0) {
+        const int dq = kWave / ng, dr = kWave - dq * ng;
+        int o = lane / ng, g = lane - (lane / ng) * ng;
+        for (int p0 = 0; p0 < total; p0 += kWave) {
+            bool dirty = false;
+            if (o < no) {
+                const float4 c = CW[g];
+                dirty = !coarse_clear(M, o, c, c.w + cut + kCullPad);
+            }
+            const unsigned long long mask = __ballot(dirty);
+            if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = (o << 6) | g;
+            qn += static_cast<int>(__popcll(mask));
+            o += dq;
+            g += dr;
+            if (g >= ng) { g -= ng; ++o; }
+            if (qn >= flush_at) {
+                if (refine()) {
+                    flops += coarse_flops(min(p0 + kWave, total));
+                    return done(true);
+                }
+                flush_at = min(2 * flush_at, kWave);
+            }
         }
-        const long long p1 = min(p0 + kWave, total);
-        flops += pairs_in(p0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(p0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
-                 pairs_in(p0, p1, ng, nwh + nwb, nwh + nwb + nwc) * kFlopsSC +
-                 pairs_in(p0, p1, ng, nwh + nwb + nwc, no) * kFlopsSS;
-        const unsigned long long mask = __ballot(dirty);
-        if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
-        qn += static_cast<int>(__popcll(mask));
-        if (qn >= flush_at) {
-            if (refine()) return done(true);
-            flush_at = min(2 * flush_at, kWave);
-        }
+        flops += coarse_flops(total);
     }
     if (qn > 0 && refine()) return done(true);
     for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {

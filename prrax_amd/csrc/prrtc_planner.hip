@@ -401,7 +401,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qpt(list[c0 + c], d) : 0.0f;
             }
             __syncthreads();
-            nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, qsh, red_d, red_i);
+            nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i));
             if (tid < nc) gmin64(&keys[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
             __syncthreads();
         }
@@ -462,7 +462,9 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                  size_a, size_b);
         team_barrier(team);  // B1: nearest neighbours of all explorers are in global memory
         mark(0);
-        // ---- phase 1c: dynamic-domain skip and steer, every explorer (redundant per member)
+        // ---- phase 1c: dynamic-domain skip and steer, every explorer (redundant per member).
+        // The nearest node's radius and coordinates are loaded together (one memory latency, not
+        // dim + 1); the coordinates stay in vec (free until the connect prep) for phase 2.
         for (int e = tid; e < S; e += nt) {
             const int t = side[e];
             const float* T = t ? TB : TA;
@@ -470,15 +472,22 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const unsigned long long key = gload64(&V.g_key[e]);
             const int ni = static_cast<int>(key & 0xffffffffull);
             nn[e] = ni;
+            float base[MAXD];
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) base[d] = ldc<COH>(T + static_cast<size_t>(min(d, dim - 1)) * ncap + ni);
+            const float rad = a.dd ? ldc<COH>(R + ni) : 0.0f;
             const float nd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
-            if (a.dd && ldc<COH>(R + ni) < nd) {
+            if (a.dd && rad < nd) {
                 state[e] = 0;
             } else {
                 const float s = (nd > 1e-8f) ? fminf(1.0f, a.step_size / nd) : 1.0f;
-                for (int d = 0; d < dim; ++d) {
-                    const float smp = sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r);
-                    const float base = ldc<COH>(T + static_cast<size_t>(d) * ncap + ni);
-                    cand[e * dim + d] = base + (smp - base) * s;
+#pragma unroll
+                for (int d = 0; d < MAXD; ++d) {
+                    if (d < dim) {
+                        const float smp = sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r);
+                        cand[e * dim + d] = base[d] + (smp - base[d]) * s;
+                        vec[e * dim + d] = base[d];
+                    }
                 }
                 state[e] = 1;
             }
@@ -536,17 +545,17 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 }
                 const int nk = min(ku, G - k0);
                 if (lane < nk) {
-                    const float* T = side[e] ? TB : TA;
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
                     for (int d = 0; d < dim; ++d) {
-                        const float base = ldc<COH>(T + static_cast<size_t>(d) * ncap + nn[e]);
+                        const float base = vec[e * dim + d];  // the nearest node (phase 1c)
                         L.q[d * fbp] = base + tf * (cand[e * dim + d] - base);
                     }
                     if (M.enabled) fk_lane(M, L);
                 }
                 wave_sync();
                 for (int c = 0; c < nk; ++c) {
-                    if (uniform(gload(&V.g_fail[e]))) break;
+                    // (the flag was read when the unit was taken; re-read from the second waypoint)
+                    if (c > 0 && uniform(gload(&V.g_fail[e]))) break;
                     if (lane == 0) ++evals;
                     if (wave_in_collision(M, WA.T + c, fbp, flops)) {
                         if (lane == 0) gstore(&V.g_fail[e], 1);
@@ -632,14 +641,17 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const unsigned long long key = gload64(&V.g_okey[e]);
             const int oi = static_cast<int>(key & 0xffffffffull);
             onn[e] = oi;
+            float tgt[MAXD];  // the other tree's nearest node, loaded together
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) tgt[d] = ldc<COH>(O + static_cast<size_t>(min(d, dim - 1)) * ncap + oi);
             const float cd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
             int n_ext = 0;
             if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
             next[e] = n_ext;
-            for (int d = 0; d < dim; ++d)
-                vec[e * dim + d] = (n_ext > 0)
-                    ? (ldc<COH>(O + static_cast<size_t>(d) * ncap + oi) - cand[e * dim + d]) / static_cast<float>(n_ext)
-                    : 0.0f;
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d)
+                if (d < dim)
+                    vec[e * dim + d] = (n_ext > 0) ? (tgt[d] - cand[e * dim + d]) / static_cast<float>(n_ext) : 0.0f;
         }
         __syncthreads();
         // ---- phase 5: connect walks.  Work unit = ku consecutive waypoints of sub-edge j of one
@@ -729,7 +741,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 }
                 wave_sync();
                 for (int c = 0; c < nk; ++c) {
-                    if (jj >= uniform(gload(&V.g_cfail[e]))) break;
+                    // (the cap was read just before the unit; re-read from the second waypoint)
+                    if (c > 0 && jj >= uniform(gload(&V.g_cfail[e]))) break;
                     if (lane == 0) ++evals;
                     if (wave_in_collision(M, WA.T + c, fbp, flops)) {
                         if (lane == 0) gmin(&V.g_cfail[e], jj);
