@@ -94,7 +94,7 @@ struct PlanArgs {
     int* qstate;                 // [B][QS_COUNT]
     int* ctl;                    // [kCtl]
     int* cont;                   // [3][B] continue lists
-    unsigned* tsync;             // [slots + 1][2] (arrivals, generation); the last pair is the grid's
+    unsigned* tsync;             // team barrier lines (make_team); the last team is the grid
     unsigned long long* stats;   // [4]
     // scheduling
     int pass1_rounds;            // round budget of pass 1 (0: pass 1 only initialises)
@@ -103,7 +103,7 @@ struct PlanArgs {
     int slots;
     // LDS layout
     ModelLayout ml;
-    size_t off_expl, off_cand, off_primes, off_nn, off_hash, off_misc;
+    size_t off_expl, off_cand, off_primes, off_sampler, off_nn, off_hash, off_misc;
     int hash_size;
     float wall_khz;
 };
@@ -122,21 +122,10 @@ enum { SC_QUERY = 0, SC_SIZE_A, SC_SIZE_B, SC_STATUS, SC_EVENT, SC_OVERFLOW, SC_
        SC_COUNT = 16 };
 
 // ---------------------------------------------------------------------------------- sampler
-__device__ __forceinline__ float halton(int base, int index) {
-    float fraction = 1.0f, result = 0.0f;
-    int n = index;
-    while (n > 0) {
-        fraction /= static_cast<float>(base);
-        result += fraction * static_cast<float>(n % base);
-        n /= base;
-    }
-    return result;
-}
-
+// Explorer e's order of the 16 Halton bases {3, 5, ..., 59}, as indices into that list.
 __device__ __forceinline__ void prime_order(int e, unsigned seed, unsigned char* p) {
-    const unsigned char P[16] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};
 #pragma unroll
-    for (int k = 0; k < 16; ++k) p[k] = P[k];
+    for (int k = 0; k < 16; ++k) p[k] = static_cast<unsigned char>(k);
     if (e != 0 || seed != 0u) {
         unsigned rng = (static_cast<unsigned>(e) + seed * 65536u) * 1664525u + 1013904223u;
         for (int i = 15; i > 0; --i) {
@@ -149,10 +138,62 @@ __device__ __forceinline__ void prime_order(int e, unsigned seed, unsigned char*
     }
 }
 
-// Explorer e's sample of round r in dimension d (prrtc_planner.cu:783-788).
-__device__ __forceinline__ float sample_dim(const float* lo, const float* hi, const unsigned char* pe, int d, int r) {
-    const float val = halton(pe[d & 15], r);
-    return lo[d] + val * (hi[d] - lo[d]);
+// The Halton radical inverse of index n in base b (prrtc_planner.cu:144-157): fraction = 1;
+// while n > 0 { fraction /= b; result += fraction * (n % b); n /= b }.  Per-work-group tables
+// in LDS make it a short loop (sampler_bytes() at one 16-byte aligned address):
+//   frac[16][kHaltonDigits]  frac[k][j] is the fraction that multiplies digit j in base k,
+//                            b^-(j+1) by the same repeated correctly rounded division, so the
+//                            products and sums of sample_dim are bit-identical to the loop's;
+//   magic[16] (u64)          floor(2^64 / b) + 1: q = high 64 bits of n * magic is n / b exactly
+//                            for every 32-bit n (the error n * (magic * b - 2^64) / b < 1 / b);
+//   base[16], lo[kMaxDim], span[kMaxDim] = hi - lo.
+constexpr int kHaltonDigits = 20;  // 3^20 > 2^31: every digit of a non-negative int index
+constexpr int kSmpMagic = 16 * kHaltonDigits * 4, kSmpBase = kSmpMagic + 16 * 8, kSmpLo = kSmpBase + 16 * 4,
+              kSmpSpan = kSmpLo + kMaxDim * 4;
+__host__ __device__ inline size_t sampler_bytes() { return align16(kSmpSpan + kMaxDim * 4); }
+
+// Fills the tables (every thread calls it; a __syncthreads must follow before use).  The tables
+// depend on the batch's limits only: each planner kernel fills them once, in its prologue.
+__device__ __forceinline__ void sampler_init(char* tab, const float* lo, const float* hi, int dim) {
+    const int tid = threadIdx.x;
+    if (tid < 16) {
+        const unsigned char P[16] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};
+        unsigned b = 3;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (tid == k) b = P[k];
+        float f = 1.0f;
+        float* F = reinterpret_cast<float*>(tab) + tid * kHaltonDigits;
+        for (int j = 0; j < kHaltonDigits; ++j) {
+            f /= static_cast<float>(b);
+            F[j] = f;
+        }
+        reinterpret_cast<unsigned long long*>(tab + kSmpMagic)[tid] = ~0ull / b + 1ull;  // b odd
+        reinterpret_cast<unsigned*>(tab + kSmpBase)[tid] = b;
+    }
+    if (tid < dim) {
+        reinterpret_cast<float*>(tab + kSmpLo)[tid] = lo[tid];
+        reinterpret_cast<float*>(tab + kSmpSpan)[tid] = hi[tid] - lo[tid];
+    }
+}
+
+// Explorer e's sample of round r in dimension d (prrtc_planner.cu:783-788); pe is the explorer's
+// base order from prime_order: lo[d] + radical_inverse(r, base) * (hi[d] - lo[d]).
+__device__ __forceinline__ float sample_dim(const char* tab, const unsigned char* pe, int d, int r) {
+    const int k = pe[d & 15];
+    const unsigned long long mg = reinterpret_cast<const unsigned long long*>(tab + kSmpMagic)[k];
+    const unsigned b = reinterpret_cast<const unsigned*>(tab + kSmpBase)[k];
+    const float* F = reinterpret_cast<const float*>(tab) + k * kHaltonDigits;
+    const unsigned long long mlo = mg & 0xffffffffull, mhi = mg >> 32;
+    float result = 0.0f;
+    unsigned n = static_cast<unsigned>(r);
+    for (int j = 0; n > 0; ++j) {
+        const unsigned long long t = ((static_cast<unsigned long long>(n) * mlo) >> 32) + static_cast<unsigned long long>(n) * mhi;
+        const unsigned q = static_cast<unsigned>(t >> 32);
+        result += F[j] * static_cast<float>(n - q * b);
+        n = q;
+    }
+    return reinterpret_cast<const float*>(tab + kSmpLo)[d] + result * reinterpret_cast<const float*>(tab + kSmpSpan)[d];
 }
 
 // ---------------------------------------------------------------------------------- helpers
@@ -226,10 +267,26 @@ __device__ __forceinline__ int unit_size(long long work, int waves, int fb) {
     return ku;
 }
 
+// Team barrier words.  Every counter has a 256-byte line of its own, so arrivals on one counter
+// do not queue behind another's, and the generation word that waiting work-groups poll is apart
+// from the counters.  Team t (t = slots: the grid) owns lines 2t (arrivals) and 2t + 1
+// (generation); a team of W >= kGroupMinW work-groups also owns up to kSyncGroups group lines
+// in a second area (make_team).
+constexpr int kSyncLine = 64;     // words per line
+constexpr int kSyncGroups = 16;   // group counters of a large team
+constexpr int kGroupMinW = 128;   // smallest team that arrives in groups
+__host__ __device__ inline int sync_group_count(int W) {  // a divisor of W (the grid's W need not be 2^k)
+    int G = W >= 2 * kGroupMinW ? kSyncGroups : (W >= kGroupMinW ? kSyncGroups / 2 : 1);
+    while (G > 1 && W % G != 0) G >>= 1;
+    return G;
+}
+
 // Team of W work-groups working on one query.
 struct Team {
     int W, m;
-    unsigned* sync;  // [2]: arrivals, generation
+    unsigned* sync;  // line pair: arrivals (top counter), generation (sync + kSyncLine)
+    unsigned* grp;   // group counter lines (G > 1)
+    int G;           // 1, or the groups members arrive in (member m in group m % G)
     int* error;
     unsigned gen;
 };
@@ -240,21 +297,31 @@ __device__ __forceinline__ void drain_memory() { asm volatile("s_waitcnt vmcnt(0
 // Barrier over the team's work-groups (a plain work-group barrier when W == 1).  Everything the
 // members share goes through agent-scope atomics (ldc/stc<true>, gload/gstore), so it suffices
 // that each thread's accesses have completed before its work-group counts in: relaxed counter
-// and generation words, no L2 write-back or invalidate (a team may span XCDs).  The last
-// arrival resets the counter, waits for that store, then opens the next generation.  Waiting is
-// bounded (seconds), so a broken team raises the error flag instead of hanging.
+// and generation words, no L2 write-back or invalidate (a team may span XCDs).  A large team
+// arrives in G groups: the last arrival of a group resets the group counter, waits for that
+// store and counts in at the top counter, so no counter sees more than max(W / G, G) arrivals
+// (atomics on one address are serialised).  The last arrival at the top resets it, waits for
+// that store, then opens the next generation; a member can only reach the next barrier after
+// that, when every reset has completed.  Waiting is bounded (seconds), so a broken team raises
+// the error flag instead of hanging.
+__device__ __forceinline__ bool count_in(unsigned* cnt, unsigned n) {
+    const unsigned prev = __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)cnt, 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != n - 1u) return false;
+    __hip_atomic_store((PRRTC_GLOBAL unsigned*)cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    drain_memory();
+    return true;
+}
 __device__ __forceinline__ void team_barrier(Team& t) {
     if (t.W > 1) {
         drain_memory();
         __syncthreads();
         if (threadIdx.x == 0) {
-            unsigned* cnt = t.sync;
-            unsigned* gen = t.sync + 1;
-            const unsigned prev = __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)cnt, 1u, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == static_cast<unsigned>(t.W) - 1u) {
-                __hip_atomic_store((PRRTC_GLOBAL unsigned*)cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                drain_memory();
+            unsigned* gen = t.sync + kSyncLine;
+            bool last = true;
+            if (t.G > 1) last = count_in(t.grp + (t.m % t.G) * kSyncLine, static_cast<unsigned>(t.W / t.G));
+            if (last) last = count_in(t.sync, static_cast<unsigned>(t.G > 1 ? t.G : t.W));
+            if (last) {
                 __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 unsigned spins = 0;
@@ -265,6 +332,8 @@ __device__ __forceinline__ void team_barrier(Team& t) {
                         gstore(t.error, 1);
                         break;
                     }
+                    // Another work-group gave up on a barrier: the run has failed, stop waiting.
+                    if ((spins & 1023u) == 0u && gload(t.error)) break;
                 }
             }
             drain_memory();
@@ -455,10 +524,15 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 gstore(&V.g_cfail[e], 0x7fffffff);
             }
         }
+        // This round's samples of every explorer, into cand (free until phase 1c steers them).
+        for (int k = tid; k < S * dim; k += nt) {
+            const int e = k / dim;
+            cand[k] = sample_dim(lds + a.off_sampler, primes + 16 * e, k - e * dim, r);
+        }
         __syncthreads();
         // ---- phase 1b: nearest neighbour of each explorer's sample in its tree
         nn_phase([&](int e) { return side[e]; },
-                 [&](int e, int d) { return sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r); }, V.g_key,
+                 [&](int e, int d) { return cand[e * dim + d]; }, V.g_key,
                  size_a, size_b);
         team_barrier(team);  // B1: nearest neighbours of all explorers are in global memory
         mark(0);
@@ -484,7 +558,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
 #pragma unroll
                 for (int d = 0; d < MAXD; ++d) {
                     if (d < dim) {
-                        const float smp = sample_dim(a.min_vals, a.max_vals, primes + 16 * e, d, r);
+                        const float smp = cand[e * dim + d];  // the sample
                         cand[e * dim + d] = base[d] + (smp - base[d]) * s;
                         vec[e * dim + d] = base[d];
                     }
@@ -960,11 +1034,30 @@ __device__ void init_query(const PlanArgs& a, const QueryView& V) {
 }
 
 // Pass 1: one work-group per query from the device queue.
+// Barrier words of team t (t = slots: the grid).  Group areas: teams of W >= kGroupMinW number
+// at most kMaxSlots / kGroupMinW; the grid takes the area after them.
+constexpr int kMaxSlots = 8192;
+constexpr int kGroupTeams = kMaxSlots / kGroupMinW + 1;
+__host__ __device__ inline size_t sync_words() {
+    return static_cast<size_t>(kSyncLine) * (2 * (kMaxSlots + 1) + static_cast<size_t>(kSyncGroups) * kGroupTeams);
+}
+__device__ __forceinline__ Team make_team(const PlanArgs& a, int t, int W, int m) {
+    Team team{W, m, a.tsync + static_cast<size_t>(2 * kSyncLine) * t, nullptr, sync_group_count(W), a.ctl + CTL_ERROR,
+              0u};
+    if (team.G > 1) {
+        const int gt = (t == a.slots) ? kGroupTeams - 1 : t;  // a grouped team's index is < slots / 128
+        team.grp = a.tsync + static_cast<size_t>(2 * kSyncLine) * (kMaxSlots + 1) +
+                   static_cast<size_t>(kSyncGroups * kSyncLine) * gt;
+    }
+    return team;
+}
+
 template <int MAXD>
 __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
     int* sc = reinterpret_cast<int*>(lds + a.off_misc) + 20;
-    Team team{1, 0, nullptr, a.ctl + CTL_ERROR, 0u};
+    Team team{1, 0, nullptr, nullptr, 1, a.ctl + CTL_ERROR, 0u};
+    sampler_init(lds + a.off_sampler, a.min_vals, a.max_vals, a.dim);  // batch-wide; synced below
     for (;;) {
         if (threadIdx.x == 0) sc[SC_QUERY] = global_add(a.ctl + CTL_QUEUE, 1);
         __syncthreads();
@@ -988,7 +1081,8 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_ncont;
     const int b = blockIdx.x;
-    Team grid{a.slots, b, a.tsync + 2 * a.slots, a.ctl + CTL_ERROR, 0u};
+    Team grid = make_team(a, a.slots, a.slots, b);
+    sampler_init(lds + a.off_sampler, a.min_vals, a.max_vals, a.dim);  // batch-wide; synced below
     for (int ep = 0;; ++ep) {
         const int cur = ep % 3, nxt = (ep + 1) % 3;
         if (threadIdx.x == 0) s_ncont = gload(&a.ctl[CTL_NC + cur]);
@@ -1012,10 +1106,9 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
         // teams re-form (and grow) while most queries of a batch finish.
         const int budget = (a.epoch_rounds > 0) ? min(a.epoch_rounds, 4 << min(ep, 20)) : -1;
         if (team_id < nteams && team_id < ncont) {
-            unsigned* ts = a.tsync + 2 * team_id;
-            Team team{W, member, ts, a.ctl + CTL_ERROR, 0u};
-            if (threadIdx.x == 0)  // the pair is at rest: every barrier on it ended before the grid barrier
-                team.gen = __hip_atomic_load((PRRTC_GLOBAL unsigned*)(ts + 1), __ATOMIC_ACQUIRE,
+            Team team = make_team(a, team_id, W, member);
+            if (threadIdx.x == 0)  // the words are at rest: every barrier on them ended before the grid barrier
+                team.gen = __hip_atomic_load((PRRTC_GLOBAL unsigned*)(team.sync + kSyncLine), __ATOMIC_ACQUIRE,
                                              __HIP_MEMORY_SCOPE_AGENT);
             for (int k = team_id; k < ncont; k += nteams) {
                 const unsigned long long t0 = wall_clock64();
@@ -1029,20 +1122,18 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     }
 }
 
-__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, int nsync,
-                             unsigned long long shape) {
+__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, unsigned long long shape) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < kCtl) ctl[t] = 0;
     if (t < 16) stats[t] = (t == 3) ? shape : 0ull;
-    for (int i = t; i < 2 * nsync; i += gridDim.x * blockDim.x) tsync[i] = 0u;
+    const int lines = static_cast<int>(sync_words() / kSyncLine);
+    for (int i = t; i < lines; i += gridDim.x * blockDim.x) tsync[static_cast<size_t>(i) * kSyncLine] = 0u;
 }
 
 // ---------------------------------------------------------------------------------- host side
 struct WsLayout {
     size_t radii, xg, qstate, ctl, cont, tsync, stats, total;
 };
-
-constexpr int kMaxSlots = 8192;
 
 WsLayout ws_layout(int B, int ncap, int S) {
     WsLayout w;
@@ -1052,7 +1143,8 @@ WsLayout ws_layout(int B, int ncap, int S) {
     w.qstate = o; o = align16(o + sizeof(int) * QS_COUNT * static_cast<size_t>(B));
     w.ctl = o; o = align16(o + sizeof(int) * kCtl);
     w.cont = o; o = align16(o + sizeof(int) * 3 * static_cast<size_t>(B));
-    w.tsync = o; o = align16(o + sizeof(unsigned) * 2 * (kMaxSlots + 1));
+    o = (o + 255) & ~static_cast<size_t>(255);  // barrier lines start on a 256-byte boundary
+    w.tsync = o; o = align16(o + sizeof(unsigned) * sync_words());
     w.stats = o; o = align16(o + 16 * sizeof(unsigned long long));
     w.total = o + 256;
     return w;
@@ -1063,6 +1155,7 @@ size_t planner_lds(const ModelLayout& ml, int S, int dim, PlanArgs& a) {
     a.off_expl = o; o = align16(o + expl_bytes(S));
     a.off_cand = o; o = align16(o + sizeof(float) * 2 * static_cast<size_t>(S) * dim);
     a.off_primes = o; o = align16(o + 16 * static_cast<size_t>(S));
+    a.off_sampler = o; o = align16(o + sampler_bytes());
     a.off_nn = o; o = align16(o + nn_bytes());
     a.hash_size = hash_size_for(S);
     a.off_hash = o; o = align16(o + 3 * sizeof(int) * static_cast<size_t>(a.hash_size));
@@ -1195,7 +1288,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
                                      (static_cast<unsigned long long>(nt) << 16) |
                                      (static_cast<unsigned long long>(a.pass1_rounds & 0xffff) << 32) |
                                      (static_cast<unsigned long long>(a.wmax & 0xffff) << 48);
-    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, a.slots + 1, shape);
+    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, shape);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
     e = small ? launch_passes<8>(a, nt, lds, stream) : launch_passes<16>(a, nt, lds, stream);

@@ -81,7 +81,7 @@ def test_cost_matches_torch_autograd_and_custom_function():
         ref = torch_sphere_cost64(cc, q64)
         ref.backward()
         want = (k + 1) * q64.grad.numpy()
-        assert abs(float(c[k]) - float(ref)) <= 1e-4 * abs(float(ref)) + 1e-6
+        assert abs(float(c[k].detach()) - float(ref.detach())) <= 1e-4 * abs(float(ref.detach())) + 1e-6
         assert np.abs(got[k] - want).max() <= 2e-3 * max(1.0, np.abs(want).max()), (got[k], want)
 
 
