@@ -334,15 +334,27 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
     // so summing all MAXD terms gives the dim-term sum exactly, without a branch per term.  The
     // query rows are re-read (LDS broadcast) for every node: qoff is an opaque zero that
     // keeps the compiler from hoisting all kEC * MAXD query values into registers.
+    // For up to 8 dimensions the next node's coordinates are loaded before this node is scored
+    // (its memory latency hides behind the arithmetic); load indices are clamped into [begin, end).
     int qoff = 0;
+    float xn[MAXD];
+    auto load_node = [&](int i) {
+        const int il = min(i, end - 1);
+#pragma unroll
+        for (int d = 0; d < MAXD; ++d) {
+            const float v = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
+            xn[d] = (d < dim) ? v : 0.0f;
+        }
+    };
+    constexpr bool kAhead = MAXD <= 8;  // at 16 dimensions the second copy costs the caller registers
+    if (kAhead && begin + tid < end) load_node(begin + tid);
     for (int i = begin + tid; i < end; i += nt) {
         asm volatile("" : "+v"(qoff));
         float x[MAXD];
+        if (!kAhead) load_node(i);
 #pragma unroll
-        for (int d = 0; d < MAXD; ++d) {
-            const float v = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + i);
-            x[d] = (d < dim) ? v : 0.0f;
-        }
+        for (int d = 0; d < MAXD; ++d) x[d] = xn[d];
+        if (kAhead && i + nt < end) load_node(i + nt);
 #pragma unroll
         for (int c = 0; c < kEC; ++c) {
             const PRRTC_LDS float* qr = qsh + c * kMaxDim + qoff;
@@ -355,26 +367,20 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
             if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
         }
     }
-    // Minimum over the work-group of the keys (squared distance bits, index): distances are
-    // non-negative, so the key order is lex_less's.  Level-major, so that the kEC shuffle chains
-    // of a level are independent and overlap; the minimum is a select, not a branch.
-    unsigned long long key[kEC];
+    // Wave minimum in lex_less order: the smallest distance (distances are non-negative, so
+    // their bit patterns order as unsigned integers; the device library's wave reduction), then
+    // the smallest index among the lanes holding it (one lane, unless distances tie exactly).
 #pragma unroll
-    for (int c = 0; c < kEC; ++c)
-        key[c] = (static_cast<unsigned long long>(__float_as_uint(bd[c])) << 32) | static_cast<unsigned>(bi[c]);
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-#pragma unroll
-        for (int c = 0; c < kEC; ++c) {
-            const unsigned long long ok = __shfl_xor(key[c], o, kWave);
-            key[c] = (ok < key[c]) ? ok : key[c];
-        }
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int c = 0; c < kEC; ++c) {
-            red_d[c * 16 + wid] = __uint_as_float(static_cast<unsigned>(key[c] >> 32));
-            red_i[c * 16 + wid] = static_cast<int>(key[c] & 0xffffffffull);
+    for (int c = 0; c < kEC; ++c) {
+        const unsigned bits = __float_as_uint(bd[c]);
+        const unsigned mbits = __ockl_wfred_min_u32(bits);
+        const unsigned long long tie = __ballot(bits == mbits);
+        int idx;
+        if (__popcll(tie) == 1) idx = __builtin_amdgcn_readlane(bi[c], static_cast<int>(__ffsll(static_cast<long long>(tie)) - 1));
+        else idx = static_cast<int>(__ockl_wfred_min_u32(bits == mbits ? static_cast<unsigned>(bi[c]) : 0xffffffffu));
+        if (lane == 0) {
+            red_d[c * 16 + wid] = __uint_as_float(mbits);
+            red_i[c * 16 + wid] = idx;
         }
     }
     __syncthreads();

@@ -146,11 +146,14 @@ __device__ __forceinline__ void prime_order(int e, unsigned seed, unsigned char*
 //                            products and sums of sample_dim are bit-identical to the loop's;
 //   magic[16] (u64)          floor(2^64 / b) + 1: q = high 64 bits of n * magic is n / b exactly
 //                            for every 32-bit n (the error n * (magic * b - 2^64) / b < 1 / b);
-//   base[16], lo[kMaxDim], span[kMaxDim] = hi - lo.
+//   base[16], lo[kMaxDim], span[kMaxDim] = hi - lo;
+//   hv[16]                   this round's radical inverse in each base (halton_round): the
+//                            index r is the round, common to all explorers, so a round needs
+//                            16 radical inverses, not one per explorer and dimension.
 constexpr int kHaltonDigits = 20;  // 3^20 > 2^31: every digit of a non-negative int index
 constexpr int kSmpMagic = 16 * kHaltonDigits * 4, kSmpBase = kSmpMagic + 16 * 8, kSmpLo = kSmpBase + 16 * 4,
-              kSmpSpan = kSmpLo + kMaxDim * 4;
-__host__ __device__ inline size_t sampler_bytes() { return align16(kSmpSpan + kMaxDim * 4); }
+              kSmpSpan = kSmpLo + kMaxDim * 4, kSmpHv = kSmpSpan + kMaxDim * 4;
+__host__ __device__ inline size_t sampler_bytes() { return align16(kSmpHv + 16 * 4); }
 
 // Fills the tables (every thread calls it; a __syncthreads must follow before use).  The tables
 // depend on the batch's limits only: each planner kernel fills them once, in its prologue.
@@ -177,10 +180,11 @@ __device__ __forceinline__ void sampler_init(char* tab, const float* lo, const f
     }
 }
 
-// Explorer e's sample of round r in dimension d (prrtc_planner.cu:783-788); pe is the explorer's
-// base order from prime_order: lo[d] + radical_inverse(r, base) * (hi[d] - lo[d]).
-__device__ __forceinline__ float sample_dim(const char* tab, const unsigned char* pe, int d, int r) {
-    const int k = pe[d & 15];
+// This round's radical inverses hv[k] of r in base k (threads k < 16; a __syncthreads must
+// follow before sample_dim reads them).
+__device__ __forceinline__ void halton_round(char* tab, int r) {
+    const int k = threadIdx.x;
+    if (k >= 16) return;
     const unsigned long long mg = reinterpret_cast<const unsigned long long*>(tab + kSmpMagic)[k];
     const unsigned b = reinterpret_cast<const unsigned*>(tab + kSmpBase)[k];
     const float* F = reinterpret_cast<const float*>(tab) + k * kHaltonDigits;
@@ -193,7 +197,14 @@ __device__ __forceinline__ float sample_dim(const char* tab, const unsigned char
         result += F[j] * static_cast<float>(n - q * b);
         n = q;
     }
-    return reinterpret_cast<const float*>(tab + kSmpLo)[d] + result * reinterpret_cast<const float*>(tab + kSmpSpan)[d];
+    reinterpret_cast<float*>(tab + kSmpHv)[k] = result;
+}
+
+// Explorer e's sample of this round in dimension d (prrtc_planner.cu:783-788); pe is the
+// explorer's base order from prime_order: lo[d] + radical_inverse(r, base) * (hi[d] - lo[d]).
+__device__ __forceinline__ float sample_dim(const char* tab, const unsigned char* pe, int d) {
+    const float* f = reinterpret_cast<const float*>(tab);
+    return f[kSmpLo / 4 + d] + f[kSmpHv / 4 + pe[d & 15]] * f[kSmpSpan / 4 + d];
 }
 
 // ---------------------------------------------------------------------------------- helpers
@@ -525,10 +536,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         // This round's samples of every explorer, into cand (free until phase 1c steers them).
-        for (int k = tid; k < S * dim; k += nt) {
-            const int e = k / dim;
-            cand[k] = sample_dim(lds + a.off_sampler, primes + 16 * e, k - e * dim, r);
-        }
+        halton_round(lds + a.off_sampler, r);
+        __syncthreads();
+        for (int e = tid; e < S; e += nt)
+            for (int d = 0; d < dim; ++d) cand[e * dim + d] = sample_dim(lds + a.off_sampler, primes + 16 * e, d);
         __syncthreads();
         // ---- phase 1b: nearest neighbour of each explorer's sample in its tree
         nn_phase([&](int e) { return side[e]; },
