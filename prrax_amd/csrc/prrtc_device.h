@@ -561,18 +561,19 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         float tx, ty, tz;
         frame_from(T, ts, M.gframe[g], R, tx, ty, tz);
         const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
-        CW[g] = make_float4(p.x, p.y, p.z, c.w);
+        CW[g] = make_float4(p.x, p.y, p.z, c.w + cut + kCullPad);  // the coarse limit, less |r_obstacle|
     }
     flops += static_cast<unsigned long long>(ng) * kFlopsXf;
     wave_sync();
     const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
-    const int no = nws + nwc + nwb + nwh;
-    const int total = no * ng;  // world primitives are staged in LDS: far below 2^31 / 64
-    // Flops of the coarse pairs [0, p1) (counted once, where the scan stops).
+    const int no = nwh + nwb + nwc;          // halfspaces, boxes, capsules; spheres follow
+    const int full = (nws / kWave) * kWave;  // spheres in whole chunks of 64 (sphere steps)
+    const int npo = no + (nws - full);       // pair-step obstacles: no, then the last nws % 64 spheres
+    const int total = npo * ng;              // world primitives are staged in LDS: far below 2^31 / 64
+    // Flops of the first p1 pair steps' pairs (counted once, where the scan stops).
     auto coarse_flops = [&](int p1) {
         return pairs_in(0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
-               pairs_in(0, p1, ng, nwh + nwb, nwh + nwb + nwc) * kFlopsSC +
-               pairs_in(0, p1, ng, nwh + nwb + nwc, no) * kFlopsSS;
+               pairs_in(0, p1, ng, nwh + nwb, no) * kFlopsSC + pairs_in(0, p1, ng, no, npo) * kFlopsSS;
     };
     int qn = 0;
     // Refine as soon as one pair is queued, then wait for twice as many after each clear
@@ -595,34 +596,67 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         wave_sync();
         return any;
     };
-    // Pair p = o * ng + g (obstacle-major); lane l takes p0 + l, and (o, g) advance by kWave
-    // pairs per step without a division (ng <= 64).
+    auto push = [&](bool dirty, int entry) {
+        const unsigned long long mask = __ballot(dirty);
+        if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
+        qn += static_cast<int>(__popcll(mask));
+    };
+    // Pair steps (halfspaces, boxes, capsules, then the spheres past the last whole chunk): pair
+    // p = o * ng + g (obstacle-major), lane l takes p0 + l, and (o, g) advance by kWave pairs per
+    // step without a division (ng <= 64).  Sphere steps (the many): lanes over a chunk of 64 world
+    // spheres, each loaded once, one group per step: one distance test per lane, no per-lane type
+    // dispatch (coarse_clear's sphere test).  Steps fill the queue up to flush_at; one refinement
+    // site serves both.
+    const int psteps = (total + kWave - 1) / kWave;
+    const int ssteps = (full / kWave) * ng;
+    int o = 0, g = 0, dq = 0, dr = 0;
     if (total > 0) {
-        const int dq = kWave / ng, dr = kWave - dq * ng;
-        int o = lane / ng, g = lane - (lane / ng) * ng;
-        for (int p0 = 0; p0 < total; p0 += kWave) {
+        dq = kWave / ng;
+        dr = kWave - dq * ng;
+        o = lane / ng;
+        g = lane - (lane / ng) * ng;
+    }
+    int pstep = 0, sstep = 0, sg = 0, k0 = 0;
+    float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float rw = 0.0f;
+    for (;;) {
+        for (; pstep < psteps && qn < flush_at; ++pstep) {
             bool dirty = false;
-            if (o < no) {
+            int entry = 0;
+            if (o < npo) {
+                const int og = (o < no) ? o : o + full;
                 const float4 c = CW[g];
-                dirty = !coarse_clear(M, o, c, c.w + cut + kCullPad);
+                dirty = !coarse_clear(M, og, c, c.w);
+                entry = (og << 6) | g;
             }
-            const unsigned long long mask = __ballot(dirty);
-            if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = (o << 6) | g;
-            qn += static_cast<int>(__popcll(mask));
             o += dq;
             g += dr;
             if (g >= ng) { g -= ng; ++o; }
-            if (qn >= flush_at) {
-                if (refine()) {
-                    flops += coarse_flops(min(p0 + kWave, total));
-                    return done(true);
+            push(dirty, entry);
+        }
+        if (pstep >= psteps) {
+            for (; sstep < ssteps && qn < flush_at; ++sstep) {
+                if (sg == 0) {
+                    w = M.wsph[k0 + lane];
+                    rw = fabsf(w.w);
                 }
-                flush_at = min(2 * flush_at, kWave);
+                const float4 c = CW[sg];
+                const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+                push(!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw),
+                     ((no + k0 + lane) << 6) | sg);
+                if (++sg == ng) { sg = 0; k0 += kWave; }
             }
         }
-        flops += coarse_flops(total);
+        const bool all = pstep >= psteps && sstep >= ssteps;
+        if (qn > 0 && refine()) {
+            flops += coarse_flops(min(pstep * kWave, total)) +
+                     static_cast<unsigned long long>(sstep) * kWave * kFlopsSS;
+            return done(true);
+        }
+        if (all) break;
+        flush_at = min(2 * flush_at, kWave);
     }
-    if (qn > 0 && refine()) return done(true);
+    flops += coarse_flops(total) + static_cast<unsigned long long>(ssteps) * kWave * kFlopsSS;
     for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
         const int k = p0 + lane;
         bool h = false;
