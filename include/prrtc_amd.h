@@ -117,8 +117,9 @@ typedef struct {
     int32_t threads;      /* threads per work-group, multiple of 64 (planner <= 512, checks <= 1024); 0 = auto */
     int32_t slots;        /* resident work-groups; 0 = all that fit */
     int32_t pass1_rounds; /* planner: rounds a query runs on one work-group before it is handed to a team of
-                             work-groups; -1 = auto (24 when the batch fills the resident work-groups,
-                             else 0: every query starts on a team) */
+                             work-groups (once every query of the batch has started, a query moves on
+                             after 4 rounds); -1 = auto (24 when the batch fills the resident
+                             work-groups, else 0: every query starts on a team) */
     int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (512) */
     int32_t epoch_rounds; /* planner: most rounds a query runs between two re-formings of the teams of
                              the team pass (epochs run 4, 8, 16, ... rounds up to this); 0 = auto (32) */

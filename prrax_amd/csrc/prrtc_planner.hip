@@ -23,10 +23,11 @@
 //
 // Scheduling (two launches on one stream, no host synchronisation in between):
 //   pass 1  one work-group per query, queries pulled from a device queue; a query still open
-//           after `pass1_rounds` rounds saves its round state and joins the continue list;
+//           after `pass1_rounds` rounds (or, once every query has started, after
+//           kPass1MinRounds) saves its round state and joins the continue list;
 //   pass 2  a cooperative launch: the continue list is split over teams of W work-groups
-//           (W = slots / open queries, a power of two, capped), so the few long queries of an
-//           uneven batch get the whole GPU.  A single query skips pass 1 (budget 0).
+//           (W = slots / open queries, capped), so the few long queries of an uneven batch
+//           get the whole GPU.  A single query skips pass 1 (budget 0).
 // Inside a team, explorer e belongs to member e % W; work whose result every member needs
 // (prefix sums, sizes, status) is computed redundantly from global state after a team barrier.
 #include <hip/hip_runtime.h>
@@ -119,7 +120,12 @@ __host__ __device__ inline int hash_size_for(int S) {
 }
 
 enum { SC_QUERY = 0, SC_SIZE_A, SC_SIZE_B, SC_STATUS, SC_EVENT, SC_OVERFLOW, SC_CONN0, SC_CONN1, SC_CONN2, SC_ERR,
-       SC_COUNT = 16 };
+       SC_DRAIN, SC_COUNT = 16 };
+
+// Pass 1 gives a query at least this many rounds; after that, once no query is left to start,
+// the query moves on to pass 2 (teams) instead of spending the rest of its budget on one
+// work-group while the others idle.
+constexpr int kPass1MinRounds = 4;
 
 // ---------------------------------------------------------------------------------- sampler
 // Explorer e's order of the 16 Halton bases {3, 5, ..., 59}, as indices into that list.
@@ -269,13 +275,15 @@ __device__ __forceinline__ unsigned long long nn_key(float dsq, int idx) {
 }
 constexpr unsigned long long kNoKey = ~0ull;
 
-// Waypoints per work unit of a check phase holding `work` waypoints: fb (checks in order, so a
-// collision skips the rest of the unit) unless the team has more waves than units, in which case
-// units shrink (down to one waypoint) so that every wave gets one.  Uniform over the team.
-__device__ __forceinline__ int unit_size(long long work, int waves, int fb) {
-    int ku = fb;
-    while (ku > 1 && work / ku < waves) ku >>= 1;
-    return ku;
+// Waypoints per work unit of a check phase over `edges` edges of G waypoints (a unit is ku
+// consecutive waypoints of one edge, checked in order by one wave after one FK of all of them;
+// a collision skips the rest of the unit): the smallest power of two ku <= fb whose units are no
+// more than the team's waves, so that no wave takes a second unit (a second claim and FK); fb
+// when even fb-waypoint units outnumber the waves.  Uniform over the team.
+__device__ __forceinline__ int unit_size(int edges, int G, int waves, int fb) {
+    int ku = 1;
+    while (ku < fb && static_cast<long long>(edges) * ((G + ku - 1) / ku) > waves) ku <<= 1;
+    return min(ku, fb);
 }
 
 // Team barrier words.  Every counter has a 256-byte line of its own, so arrivals on one counter
@@ -501,6 +509,11 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             break;
         }
         if (r >= r_end) break;  // round budget spent: the query goes to the next continue list
+        if (pass == 1 && r - r_begin >= kPass1MinRounds) {
+            if (tid == 0) sc[SC_DRAIN] = (gload(a.ctl + CTL_QUEUE) >= a.B) ? 1 : 0;
+            __syncthreads();
+            if (sc[SC_DRAIN]) break;  // every query has started: continue in pass 2
+        }
         const int size_a = sc[SC_SIZE_A], size_b = sc[SC_SIZE_B];
         // ---- phase 1a: tree side of every explorer (each member keeps a full copy); owners clear
         // the shared collision flags of their explorers for this round.
@@ -589,7 +602,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         // anywhere on the edge stops the remaining units of that edge).
         {
             const int nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
-            const int ku = unit_size(static_cast<long long>(nl) * G, W * nwaves, fb);
+            const int ku = unit_size(nl, G, W * nwaves, fb);
             const int cps = (G + ku - 1) / ku;
             // Fewer waypoints than waves: a fixed round-robin deal of one-waypoint units (no
             // claiming; nothing is left to skip).  Otherwise units are claimed dynamically (claim
@@ -751,7 +764,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 block_excl_scan(k < nl ? next[list[k]] : 0, scan, &tot);
                 nsub += tot;
             }
-            const int ku = unit_size(static_cast<long long>(nsub) * G, W * nwaves, fb);
+            const int ku = unit_size(nsub, G, W * nwaves, fb);
             const int cps = (G + ku - 1) / ku;
             // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers in
             // aux0).  Otherwise units are claimed dynamically, and a collision at sub-edge j lowers
@@ -1083,7 +1096,7 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
 }
 
 // Pass 2: teams of W work-groups resume the queries left open by pass 1, in epochs.  In each
-// epoch the open queries are dealt to teams (W = slots / open queries, a power of two, capped)
+// epoch the open queries are dealt to teams (W = slots / open queries, capped)
 // and run for at most 4 << epoch rounds (capped at `epoch_rounds`); the queries still open then
 // form the next epoch's list, so the teams grow as queries finish.  A grid barrier separates
 // the epochs.
@@ -1102,8 +1115,7 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
         if (ncont == 0) return;  // every work-group reads the same count after the grid barrier
         // list (ep + 2) % 3 was read in the previous epoch and is filled in the next one
         if (b == 0 && threadIdx.x == 0) gstore(&a.ctl[CTL_NC + (ep + 2) % 3], 0);
-        int W = 1;
-        while (W * 2 <= a.wmax && W * 2 * ncont <= a.slots) W *= 2;
+        const int W = max(1, min(a.wmax, a.slots / ncont));  // every resident work-group has a team
         int team_id, member;
         if (a.slots % (8 * W) == 0) {  // keep a team on one XCD (work-groups are dealt round-robin)
             team_id = (b % 8) + 8 * ((b / 8) / W);
