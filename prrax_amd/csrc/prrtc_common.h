@@ -330,41 +330,61 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
     int bi[kEC];
 #pragma unroll
     for (int c = 0; c < kEC; ++c) { bd[c] = FLT_MAX; bi[c] = 0; }
-    // Dimensions d >= dim are zero in the node (x) and in every query row, and fma(0, 0, s) == s,
-    // so summing all MAXD terms gives the dim-term sum exactly, without a branch per term.  The
-    // query rows are re-read (LDS broadcast) for every node: qoff is an opaque zero that
-    // keeps the compiler from hoisting all kEC * MAXD query values into registers.
-    // For up to 8 dimensions the next node's coordinates are loaded before this node is scored
-    // (its memory latency hides behind the arithmetic); load indices are clamped into [begin, end).
+    // Dimensions d >= dim are zero in the node and in every query row, and fma(0, 0, s) == s, so
+    // summing all MAXD terms gives the dim-term sum exactly, without a branch per term.
+    // The query rows are re-read (LDS broadcast) for every node: qoff is an opaque zero that
+    // keeps the compiler from hoisting all kEC * MAXD query values into registers.  Load indices
+    // are clamped into [begin, end).
     int qoff = 0;
-    float xn[MAXD];
-    auto load_node = [&](int i) {
+    auto load_node = [&](int i, float* xo) {
         const int il = min(i, end - 1);
 #pragma unroll
         for (int d = 0; d < MAXD; ++d) {
             const float v = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
-            xn[d] = (d < dim) ? v : 0.0f;
+            xo[d] = (d < dim) ? v : 0.0f;
         }
     };
-    constexpr bool kAhead = MAXD <= 8;  // at 16 dimensions the second copy costs the caller registers
-    if (kAhead && begin + tid < end) load_node(begin + tid);
-    for (int i = begin + tid; i < end; i += nt) {
-        asm volatile("" : "+v"(qoff));
-        float x[MAXD];
-        if (!kAhead) load_node(i);
+    if constexpr (MAXD <= 8) {
+        // Two nodes per step, i and i + nt (both loads issued together), scored against each
+        // query row read once.  A thread visits its nodes in increasing index order (i before
+        // i + nt), so on equal distances the lower index still wins.
+        for (int i = begin + tid; i < end; i += 2 * nt) {
+            asm volatile("" : "+v"(qoff));
+            float xa[MAXD], xb[MAXD];
+            load_node(i, xa);
+            load_node(i + nt, xb);
+            const bool hb = i + nt < end;
 #pragma unroll
-        for (int d = 0; d < MAXD; ++d) x[d] = xn[d];
-        if (kAhead && i + nt < end) load_node(i + nt);
+            for (int c = 0; c < kEC; ++c) {
+                const PRRTC_LDS float* qr = qsh + c * kMaxDim + qoff;
+                float da = 0.0f, db = 0.0f;
 #pragma unroll
-        for (int c = 0; c < kEC; ++c) {
-            const PRRTC_LDS float* qr = qsh + c * kMaxDim + qoff;
-            float dsq = 0.0f;
-#pragma unroll
-            for (int d = 0; d < MAXD; ++d) {
-                const float df = x[d] - qr[d];
-                dsq = __builtin_fmaf(df, df, dsq);
+                for (int d = 0; d < MAXD; ++d) {
+                    const float q = qr[d];
+                    const float ea = xa[d] - q, eb = xb[d] - q;
+                    da = __builtin_fmaf(ea, ea, da);
+                    db = __builtin_fmaf(eb, eb, db);
+                }
+                if (da < bd[c]) { bd[c] = da; bi[c] = i; }
+                if (hb && db < bd[c]) { bd[c] = db; bi[c] = i + nt; }
             }
-            if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
+        }
+    } else {
+        for (int i = begin + tid; i < end; i += nt) {
+            asm volatile("" : "+v"(qoff));
+            float x[MAXD];
+            load_node(i, x);
+#pragma unroll
+            for (int c = 0; c < kEC; ++c) {
+                const PRRTC_LDS float* qr = qsh + c * kMaxDim + qoff;
+                float dsq = 0.0f;
+#pragma unroll
+                for (int d = 0; d < MAXD; ++d) {
+                    const float df = x[d] - qr[d];
+                    dsq = __builtin_fmaf(df, df, dsq);
+                }
+                if (dsq < bd[c]) { bd[c] = dsq; bi[c] = i; }
+            }
         }
     }
     // Wave minimum in lex_less order: the smallest distance (distances are non-negative, so
