@@ -327,11 +327,14 @@ def test_paths_valid_under_fp64_checker():
                 assert d > -1e-5, f"query {i}: path point in collision (d={d})"
 
 
-@pytest.mark.parametrize("pass1_rounds,team_max,epoch_rounds",
-                         [(0, 1, 0), (0, 2, 0), (0, 8, 0), (1, 4, 0), (2, 32, 0), (-1, 0, 0), (0, 512, 1), (3, 0, 2),
-                          (1, 16, 7)])
-def test_team_schedules_bit_exact(pass1_rounds, team_max, epoch_rounds):
-    """Hand-over budget, team size and epoch length change who computes what, never the result."""
+@pytest.mark.parametrize("pass1_rounds,team_max,epoch_rounds,slots",
+                         [(0, 1, 0, 0), (0, 2, 0, 0), (0, 8, 0, 0), (1, 4, 0, 0), (2, 32, 0, 0), (-1, 0, 0, 0),
+                          (0, 512, 1, 0), (3, 0, 2, 0), (1, 16, 7, 0), (8, 0, 0, 0), (0, 0, 3, 300)])
+def test_team_schedules_bit_exact(pass1_rounds, team_max, epoch_rounds, slots):
+    """Hand-over budget, team size and epoch length change who computes what, never the result.
+    Teams are slots / open queries work-groups (102 for 5 queries on 512 slots: not a power of
+    two); a pass-1 budget above 4 rounds exercises the early hand-over once every query has
+    started; 300 slots give a grid barrier of 300 work-groups (arriving in 4 groups)."""
     robot = scenes.make_arm(0, 64, 200)
     qids = [0, 1, 2, 26, 455]  # 26 needs 158 rounds, 455 never connects within the cap
     ctxs, starts, goals, nodes = [], [], [], []
@@ -342,10 +345,10 @@ def test_team_schedules_bit_exact(pass1_rounds, team_max, epoch_rounds):
         s, g = query(scenes.Scene(robot, w), 3, qi, n_goals=ng)
         ctxs.append(cc); starts.append(s); goals.append(g); nodes.append(N)
     kw = dict(num_new_samples=128, granularity=16, max_iterations=40, min_vals=robot.lower, max_vals=robot.upper)
-    say(f"team schedule pass1_rounds={pass1_rounds} team_max={team_max} epoch_rounds={epoch_rounds}")
+    say(f"team schedule pass1_rounds={pass1_rounds} team_max={team_max} epoch_rounds={epoch_rounds} slots={slots}")
     res = prrtc_plan_batch(np.stack(starts), goals, collision_context=ctxs, max_nodes=max(nodes),
                            max_nodes_per_query=nodes, pass1_rounds=pass1_rounds, team_max=team_max,
-                           epoch_rounds=epoch_rounds, **kw)
+                           epoch_rounds=epoch_rounds, slots=slots, **kw)
     for k, qi in enumerate(qids):
         ora = _oracle_cache(qi, ctxs[k], starts[k], goals[k], nodes[k], kw)
         compare_plan(res[k], ora, f"c3 q{qi}")
