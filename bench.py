@@ -100,10 +100,8 @@ def scale_leg(n_queries: int, checker_factory, kw: dict, launch: dict):
     t0 = time.perf_counter()
     robot, worlds, starts, goals, nodes = build_c3(n_queries, checker_factory)
     log(f"scaling: {n_queries} queries built in {time.perf_counter() - t0:.1f}s")
-    order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
-                       kind="stable")
     pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
-                       max_nodes_per_query=nodes, order=order, **launch, **kw)
+                       max_nodes_per_query=nodes, **launch, **kw)
     pb.enqueue()
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -289,11 +287,9 @@ def main():
         kw["num_new_samples"] = tuned.num_new_samples
         launch.update({k: v for k, v in tuned.launch.items()})
         log(f"autotune: chose S={tuned.num_new_samples} {tuned.launch} in {time.perf_counter() - t0:.1f}s")
-    # Longest queries first (more obstacles -> more work): results do not depend on the order.
-    order = np.argsort([-(w["world_spheres"].shape[0] + 4 * w["world_boxes"].shape[0]) for w in worlds],
-                       kind="stable")
+    # Queue order: the library's default, longest predicted first (results do not depend on it).
     pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
-                       max_nodes_per_query=nodes, order=order, **launch, **kw)
+                       max_nodes_per_query=nodes, **launch, **kw)
     # ---- batched
     for w in range(args.warmup):
         t = time.perf_counter()

@@ -337,6 +337,23 @@ def prrtc_plan(
     return bufs.results()[0]
 
 
+def _default_order(starts: np.ndarray, goals: np.ndarray, gcounts, worlds) -> np.ndarray:
+    """Longest-predicted-first queue order: a query's first round connects every new node
+    towards the goal tree, so its cost grows with the start-to-nearest-goal distance and with
+    the obstacle count (boxes and capsules weighted as several spheres)."""
+    B = starts.shape[0]
+    d = np.linalg.norm(goals.astype(np.float64) - starts[:, None, :].astype(np.float64), axis=2)
+    if gcounts is not None:
+        d = np.where(np.arange(goals.shape[1])[None, :] < gcounts[:, None], d, np.inf)
+    dist = d.min(axis=1)
+    if worlds is None:
+        load = np.ones(B)
+    else:
+        load = np.array([1.0 + w["world_spheres"].shape[0] + 4.0 * w["world_boxes"].shape[0] +
+                         2.0 * w["world_capsules"].shape[0] for w in worlds])
+    return np.argsort(-(dist * load), kind="stable").astype(np.int32)
+
+
 def prepare_batch(
     start_configs,
     goal_configs,
@@ -368,7 +385,12 @@ def prepare_batch(
     team_max: int = 0,
     epoch_rounds: int = 0,
 ) -> PreparedBatch:
-    """Validates and uploads a batch; ``PreparedBatch.enqueue()`` then runs it (repeatably)."""
+    """Validates and uploads a batch; ``PreparedBatch.enqueue()`` then runs it (repeatably).
+
+    ``order`` is the order in which the one-work-group pass takes queries from its queue (a
+    permutation of the batch; results never depend on it).  By default the queries predicted
+    longest come first (see ``_default_order``), so they do not start last and set the end of
+    that pass."""
     collision_mode = _resolve_collision_mode(collision_checker)
     _lib.lib()
     starts = np.atleast_2d(as_numpy(start_configs, np.float32))
@@ -446,6 +468,8 @@ def prepare_batch(
         s = np.asarray(seeds, dtype=np.int64).reshape(B) & 0xFFFFFFFF
         sd = torch.from_numpy(s.astype(np.uint32).view(np.int32)).to(dev)
     od = None
+    if order is None and B > 1:
+        order = _default_order(starts, goals, gcounts, worlds if isinstance(collision_context, (list, tuple)) else None)
     if order is not None:
         o = as_numpy(order, np.int32).reshape(B)
         if not np.array_equal(np.sort(o), np.arange(B)):

@@ -92,3 +92,28 @@ def test_context_invariants():
     big["fk_topo_inv"] = np.arange(70, dtype=np.int32)
     with pytest.raises(ValueError, match="joint cap"):
         check_context(big, 7)
+
+
+def test_default_queue_order_longest_first():
+    """The default queue order of the one-work-group pass: a permutation, by start-to-nearest-
+    goal distance times obstacle load, descending; padded goal slots (beyond a query's goal
+    count) are ignored."""
+    from prrax_amd.prrtc import _default_order
+    starts = np.zeros((4, 2), np.float32)
+    goals = np.zeros((4, 2, 2), np.float32)
+    goals[0, 0] = [1.0, 0.0]
+    goals[1, 0] = [3.0, 0.0]
+    goals[2, 0] = [2.0, 0.0]
+    goals[3, 0] = [0.5, 0.0]
+    goals[3, 1] = [0.0, 9.0]          # padding: query 3 has one goal, so this slot is ignored
+    goals[:3, 1] = goals[:3, 0]
+    gcounts = np.array([2, 2, 2, 1], np.int32)
+    o = _default_order(starts, goals, gcounts, None)
+    assert sorted(o.tolist()) == [0, 1, 2, 3]
+    assert o.tolist() == [1, 2, 0, 3]
+    # obstacle load: query 0 (distance 1) with 10 spheres outranks query 1 (distance 3), empty world
+    def w(ns, nb=0, nc=0):
+        return {"world_spheres": np.zeros((ns, 4)), "world_boxes": np.zeros((nb, 15)),
+                "world_capsules": np.zeros((nc, 7))}
+    o = _default_order(starts, goals, gcounts, [w(10), w(0), w(0, 1), w(0, 0, 1)])
+    assert o.tolist() == [0, 2, 1, 3]  # keys 11, 10 (2 x (1 + 4)), 3, 1.5
