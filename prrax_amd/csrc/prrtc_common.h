@@ -44,7 +44,7 @@ struct ModelLayout {
 };
 
 __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, int nws, int nwc, int nwb, int nwh,
-                                                      int nwaves, int fbp) {
+                                                      int nwaves, int fbp, int ucap = 0) {
     ModelLayout L;
     size_t o = 0;
     L.joints = o; o = align16(o + sizeof(JointRec) * nj);
@@ -61,7 +61,7 @@ __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, in
     L.gm = o; o = align16(o + sizeof(int) * nrs);
     L.wq = o; o = align16(o + sizeof(float) * kMaxDim * fbp * nwaves);
     L.wT = o; o = align16(o + sizeof(float) * 7 * nj * fbp * nwaves);
-    L.cull_stride = static_cast<int>(align16(static_cast<size_t>(cull_stride_bytes(nrs, nj))));
+    L.cull_stride = static_cast<int>(align16(static_cast<size_t>(cull_stride_bytes(nrs, nj, ucap))));
     L.wcull = o; o = align16(o + static_cast<size_t>(L.cull_stride) * nwaves);
     L.end = o;
     L.fbp = fbp;

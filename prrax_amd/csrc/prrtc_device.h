@@ -446,10 +446,14 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
 // halfspaces whose normal is not unit are never culled.  Every fine test that does run is the
 // flat check's own test on the same FP32 values, so the verdict is identical bit for bit.
 constexpr float kCullPad = 1e-3f;
-constexpr int kCullQ = 2 * kWave;  // refinement queue (obstacle * 64 + group) per wave
+constexpr int kCullQ = 2 * kWave;     // refinement queue (obstacle * 64 + group) per wave
+constexpr int kUnitCap = 2 * kWave;   // candidate pairs of a sub-unit (wave_unit_check)
+constexpr int kUnitMax = 4;           // configurations per sub-unit
 
-__host__ __device__ inline int cull_stride_bytes(int nrs, int nj) {
-    return static_cast<int>(sizeof(float4)) * (nrs + nj) + static_cast<int>(sizeof(int)) * kCullQ;
+// Per-wave cull area: robot spheres, group spheres, refinement queue, and ucap candidate slots
+// (kUnitCap where wave_unit_check runs, else 0).
+__host__ __device__ inline int cull_stride_bytes(int nrs, int nj, int ucap) {
+    return static_cast<int>(sizeof(float4)) * (nrs + nj) + static_cast<int>(sizeof(int)) * (kCullQ + ucap);
 }
 
 // Obstacles of the culled check are numbered halfspaces, boxes, capsules, spheres: the few large
@@ -520,6 +524,14 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
 __device__ __forceinline__ unsigned long long pairs_in(long long p0, long long p1, int ng, int lo, int hi) {
     const long long a = max(p0, static_cast<long long>(lo) * ng), b = min(p1, static_cast<long long>(hi) * ng);
     return b > a ? static_cast<unsigned long long>(b - a) : 0ull;
+}
+
+// Flops of the first p1 pairs of the pair steps (halfspaces, boxes, capsules, then npo - no
+// spheres; npo pair-step obstacles, ng groups).
+__device__ __forceinline__ unsigned long long coarse_flops_of(const Model& M, int p1, int ng, int npo) {
+    const int nwh = M.n_wh, nwb = M.n_wb, no = M.n_wh + M.n_wb + M.n_wc;
+    return pairs_in(0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
+           pairs_in(0, p1, ng, nwh + nwb, no) * kFlopsSC + pairs_in(0, p1, ng, no, npo) * kFlopsSS;
 }
 
 template <bool MARGIN>
@@ -670,6 +682,193 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     return done(false);
 }
 
+// Verdict of a sub-unit of nu <= kUnitMax configurations whose FK frames are at T + c (c < nu):
+// k + 1 if configuration k is the first that collides, 0 if all are free, -1 if the candidate
+// list overflowed (the caller then checks them one by one).  Needs a cull area with kUnitCap
+// candidate slots.  Consecutive waypoints of one edge are close, so one coarse pass serves all
+// of them: group g's unit sphere has centre C_g (midpoint of its nu transformed centres) and
+// radius R_g + cut + pad + max_c |p_c - C_g| (wave_culled's coarse limit plus the spread).  The
+// coarse tests are 1-Lipschitz in the centre, so a pair the unit sphere clears is cleared by
+// every configuration's own group sphere (triangle inequality); the pairs it cannot clear form a
+// candidate list, and each configuration then runs wave_culled's coarse test on the candidates
+// only, its fine tests and the self pairs.  Verdicts are wave_culled's, one by one.
+template <bool MARGIN>
+__device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
+    const int lane = threadIdx.x & (kWave - 1);
+    char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
+    float4* FP = reinterpret_cast<float4*>(area);          // robot spheres (per configuration)
+    float4* CW = FP + M.n_spheres;                          // group spheres (per configuration)
+    int* Q = reinterpret_cast<int*>(CW + M.n_joints);       // refinement queue
+    int* LU = Q + kCullQ;                                   // candidate pairs of the sub-unit
+    float4* CU = FP;  // unit spheres (n_groups <= n_spheres); free until the first transform below
+    const int ng = M.n_groups, nrs = M.n_spheres;
+    const float cut = MARGIN ? M.margin : 0.0f;
+    unsigned lf = 0;
+    auto done = [&](int v) {
+        flops += wave_sum_u32(lf);
+        return v;
+    };
+    // ---- unit spheres (lane g: n_groups <= n_joints <= 64)
+    if (lane < ng) {
+        const int g = lane;
+        const float4 c = M.gcoarse[g];
+        float px[kUnitMax], py[kUnitMax], pz[kUnitMax];
+        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+#pragma unroll
+        for (int k = 0; k < kUnitMax; ++k) {
+            px[k] = 0.0f; py[k] = 0.0f; pz[k] = 0.0f;
+            if (k < nu) {
+                Mat3 R;
+                float tx, ty, tz;
+                frame_from(T + k, ts, M.gframe[g], R, tx, ty, tz);
+                const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
+                px[k] = p.x; py[k] = p.y; pz[k] = p.z;
+                lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
+                hx = fmaxf(hx, p.x); hy = fmaxf(hy, p.y); hz = fmaxf(hz, p.z);
+            }
+        }
+        const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
+        float spread = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kUnitMax; ++k) {
+            if (k < nu) {
+                const float dx = px[k] - cx, dy = py[k] - cy, dz = pz[k] - cz;
+                spread = fmaxf(spread, __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))));
+            }
+        }
+        // the spread rounded up (relative 1e-6); the coarse pad covers the rest of the rounding
+        CU[g] = make_float4(cx, cy, cz, (c.w + cut + kCullPad) + spread * 1.000001f);
+    }
+    flops += static_cast<unsigned long long>(ng) * nu * kFlopsXf;
+    wave_sync();
+    // ---- unit coarse stage: wave_culled's steps against the unit spheres, into LU
+    const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
+    const int no = nwh + nwb + nwc;
+    const int full = (nws / kWave) * kWave;
+    const int npo = no + (nws - full);
+    const int total = npo * ng;
+    int nl = 0;
+    auto push = [&](bool dirty, int entry) -> bool {
+        const unsigned long long mask = __ballot(dirty);
+        const int cnt = static_cast<int>(__popcll(mask));
+        if (nl + cnt > kUnitCap) return false;
+        if (dirty) LU[nl + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
+        nl += cnt;
+        return true;
+    };
+    if (total > 0) {
+        const int dq = kWave / ng, dr = kWave - dq * ng;
+        int o = lane / ng, g = lane - (lane / ng) * ng;
+        for (int p0 = 0; p0 < total; p0 += kWave) {
+            bool dirty = false;
+            int entry = 0;
+            if (o < npo) {
+                const int og = (o < no) ? o : o + full;
+                const float4 c = CU[g];
+                dirty = !coarse_clear(M, og, c, c.w);
+                entry = (og << 6) | g;
+            }
+            o += dq;
+            g += dr;
+            if (g >= ng) { g -= ng; ++o; }
+            if (!push(dirty, entry)) {
+                flops += coarse_flops_of(M, min(p0 + kWave, total), ng, npo);
+                return done(-1);
+            }
+        }
+        flops += coarse_flops_of(M, total, ng, npo);
+    }
+    for (int k0 = 0; k0 < full; k0 += kWave) {
+        const float4 w = M.wsph[k0 + lane];
+        const float rw = fabsf(w.w);
+        for (int sg = 0; sg < ng; ++sg) {
+            const float4 c = CU[sg];
+            const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+            flops += static_cast<unsigned long long>(kWave) * kFlopsSS;
+            if (!push(!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw),
+                      ((no + k0 + lane) << 6) | sg))
+                return done(-1);
+        }
+    }
+    wave_sync();
+    // ---- each configuration: its spheres and group spheres, the candidates, the self pairs
+    for (int u = 0; u < nu; ++u) {
+        const float* Tu = T + u;
+        for (int s0 = 0; s0 < nrs; s0 += kWave) {
+            const int s = s0 + lane;
+            bool act = false;
+            if (s < nrs) {
+                const int js = M.sphere_js[s];
+                float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (js >= 0) {
+                    Mat3 R;
+                    float tx, ty, tz;
+                    frame_from(Tu, ts, js, R, tx, ty, tz);
+                    const float4 sp = M.spheres[s];
+                    const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+                    o = make_float4(p.x, p.y, p.z, sp.w);
+                    act = true;
+                }
+                FP[s] = o;
+            }
+            flops += static_cast<unsigned long long>(__popcll(__ballot(act))) * kFlopsXf;
+        }
+        if (lane < ng) {
+            const int g = lane;
+            const float4 c = M.gcoarse[g];
+            Mat3 R;
+            float tx, ty, tz;
+            frame_from(Tu, ts, M.gframe[g], R, tx, ty, tz);
+            const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
+            CW[g] = make_float4(p.x, p.y, p.z, c.w + cut + kCullPad);
+        }
+        // per configuration, as wave_in_collision counts it
+        flops += static_cast<unsigned long long>(ng) * kFlopsXf + static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
+        wave_sync();
+        bool hit = false;
+        for (int k0 = 0; k0 < nl && !hit; k0 += kWave) {
+            bool dirty = false;
+            int entry = 0;
+            if (k0 + lane < nl) {
+                entry = LU[k0 + lane];
+                const int o = entry >> 6, g = entry & 63;
+                const float4 c = CW[g];
+                dirty = !coarse_clear(M, o, c, c.w);
+                lf += (o < nwh) ? kFlopsSH : (o < nwh + nwb) ? kFlopsSB : (o < no) ? kFlopsSC : kFlopsSS;
+            }
+            const unsigned long long mask = __ballot(dirty);
+            if (dirty) Q[static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
+            const int qn = static_cast<int>(__popcll(mask));
+            wave_sync();
+            bool h = false;
+            for (int e = lane; e < qn && !h; e += kWave) {
+                const int o = Q[e] >> 6, g = Q[e] & 63;
+                for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
+                    if (fine_test<MARGIN>(M, o, FP[M.gmem[m]], lf)) {
+                        h = true;
+                        break;
+                    }
+                }
+            }
+            hit = wave_any(h);
+            wave_sync();
+        }
+        if (hit) return done(u + 1);
+        for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
+            const int k = p0 + lane;
+            bool h = false;
+            if (k < M.n_pairs) {
+                const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
+                h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+            }
+            flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
+            if (wave_any(h)) return done(u + 1);
+        }
+        wave_sync();
+    }
+    return done(0);
+}
+
 // The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform; the
 // culled check gives the flat check's verdict (see wave_culled).
 __device__ __forceinline__ bool wave_in_collision(const Model& M, const float* T, int ts, unsigned long long& flops) {
@@ -677,6 +876,14 @@ __device__ __forceinline__ bool wave_in_collision(const Model& M, const float* T
     flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
     if (M.mode == 1) return wave_culled<true>(M, T, ts, flops);
     return wave_culled<false>(M, T, ts, flops);
+}
+
+// wave_unit_check with the mode dispatch of wave_in_collision (nu >= 1 configurations at T + c).
+__device__ __forceinline__ int wave_unit_in_collision(const Model& M, const float* T, int ts, int nu,
+                                                      unsigned long long& flops) {
+    if (!M.enabled) return 0;
+    if (M.mode == 1) return wave_unit_check<true>(M, T, ts, nu, flops);
+    return wave_unit_check<false>(M, T, ts, nu, flops);
 }
 
 }  // namespace prrtc

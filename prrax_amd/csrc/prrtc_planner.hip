@@ -597,259 +597,272 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         // computes it for all candidates here, so no team barrier is needed after the commit; a
         // single work-group searches after the commit, for the successful extends only.
         if (W > 1) connect_nn(true, size_a, size_b);
-        // ---- phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku
-        // consecutive waypoints of one edge (FK one per lane, then checks in order; a collision
-        // anywhere on the edge stops the remaining units of that edge).
-        {
-            const int nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
-            const int ku = unit_size(nl, G, W * nwaves, fb);
-            const int cps = (G + ku - 1) / ku;
-            // Fewer waypoints than waves: a fixed round-robin deal of one-waypoint units (no
-            // claiming; nothing is left to skip).  Otherwise units are claimed dynamically (claim
-            // counter per explorer), which balances the load and skips edges known to collide.
-            const bool deal = static_cast<long long>(nl) * G < W * nwaves;
-            int us = m * nwaves + wid;
-            int start = ((m * nwaves + wid) * S) / (W * nwaves);
-            for (;;) {
-                int e = -1, k0 = 0;
-                if (deal) {
-                    if (us >= nl * cps) break;
-                    e = list[us / cps];
-                    k0 = (us - (us / cps) * cps) * ku;
-                    us += W * nwaves;
-                    if (uniform(gload(&V.g_fail[e]))) continue;
-                } else {
-                    for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
-                        bool ok = false;
-                        int ex = 0;
-                        if (b0 + lane < S) {
-                            ex = start + b0 + lane;
-                            if (ex >= S) ex -= S;
-                            ok = state[ex] == 1 && gload(&V.g_fail[ex]) == 0 && gload(&V.g_eclaim[ex]) < cps;
-                        }
-                        const unsigned long long mask = __ballot(ok);
-                        if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
-                    }
-                    e = uniform(e);
-                    if (e < 0) break;
-                    start = e;
-                    int u = 0;
-                    if (lane == 0)
-                        u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_eclaim[e], 1, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                    u = uniform(__shfl(u, 0, kWave));
-                    if (u >= cps) continue;  // another wave took the last unit
-                    k0 = u * ku;
-                }
-                const int nk = min(ku, G - k0);
-                if (lane < nk) {
-                    const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
-                    for (int d = 0; d < dim; ++d) {
-                        const float base = vec[e * dim + d];  // the nearest node (phase 1c)
-                        L.q[d * fbp] = base + tf * (cand[e * dim + d] - base);
-                    }
-                    if (M.enabled) fk_lane(M, L);
-                }
-                wave_sync();
-                for (int c = 0; c < nk; ++c) {
-                    // (the flag was read when the unit was taken; re-read from the second waypoint)
-                    if (c > 0 && uniform(gload(&V.g_fail[e]))) break;
-                    if (lane == 0) ++evals;
-                    if (wave_in_collision(M, WA.T + c, fbp, flops)) {
-                        if (lane == 0) gstore(&V.g_fail[e], 1);
-                        break;
-                    }
-                }
-                wave_sync();
-            }
-        }
-        team_barrier(team);  // B3: every extend verdict is in global memory
-        mark(1);
-        // ---- phase 3: commit -- appends in explorer order (prefix sums), radius updates
-        {
-            for (int e = tid; e < S; e += nt) {
-                if (state[e] == 1 && gload(&V.g_fail[e])) state[e] = 3;  // 3: collided
-                if (e % W == m) gstore64(&V.g_key[e], kNoKey);  // read by every member before B3
-            }
-            if (tid == 0) sc[SC_OVERFLOW] = 0;
-            __syncthreads();
-            int base0 = size_a, base1 = size_b;
-            for (int e0 = 0; e0 < S; e0 += nt) {
-                const int e = e0 + tid;
-                const bool fr = e < S && state[e] == 1;
-                const int t = fr ? side[e] : 0;
-                int tot0, tot1;
-                const int p0 = block_excl_scan((fr && t == 0) ? 1 : 0, scan, &tot0);
-                const int p1 = block_excl_scan((fr && t == 1) ? 1 : 0, scan, &tot1);
-                if (e < S) {
-                    int ni = -1;
-                    if (fr) {
-                        const int idx = t ? base1 + p1 : base0 + p0;
-                        if (idx < N) {
-                            ni = idx;
-                            if (e % W == m) {
-                                float* T = t ? TB : TA;
-                                for (int d = 0; d < dim; ++d)
-                                    stc<COH>(T + static_cast<size_t>(d) * ncap + idx, cand[e * dim + d]);
-                                stc<COH>((t ? PB : PA) + idx, nn[e]);
-                                stc<COH>((t ? RB : RA) + idx, FLT_MAX);
-                            }
-                        } else {
-                            sc[SC_OVERFLOW] = 1;
-                        }
-                    }
-                    newi[e] = ni;
-                }
-                base0 += tot0;
-                base1 += tot1;
-            }
-            if (a.dd && m == 0) {
+        // ---- phases 2 (kind 0) and 5 (kind 1) share one check loop, so that the FK and the
+        // collision checks are instantiated once (the kernels' register budget); the phase-3
+        // commit and the connect prep run between them, after B3.
+        //   phase 2: extend edges, waypoint t_k = (k+1)/G, k = 0..G-1.  Work unit = ku consecutive
+        //     waypoints of one edge (FK one per lane, then checks in order; a collision anywhere
+        //     on the edge stops the remaining units of that edge).
+        //   phase 5: connect walks.  Work unit = ku consecutive waypoints of sub-edge j of one
+        //     explorer's walk; a collision at sub-edge j lowers the explorer's first failing
+        //     sub-edge g_cfail[e] (an order-independent minimum), and units past it are skipped.
+        bool ended = false;  // phase 3 ended the query (tree full)
+#pragma clang loop unroll(disable)
+        for (int kind = 0; kind < 2; ++kind) {
+            if (kind == 1) {
+            team_barrier(team);  // B3: every extend verdict is in global memory
+            mark(1);
+            // ---- phase 3: commit -- appends in explorer order (prefix sums), radius updates
+            {
                 for (int e = tid; e < S; e += nt) {
-                    if (state[e] == 3) hash_add(hk, hs, hg, a.hash_size, nn[e] * 2 + side[e], true);
-                    else if (newi[e] >= 0) hash_add(hk, hs, hg, a.hash_size, nn[e] * 2 + side[e], false);
+                    if (state[e] == 1 && gload(&V.g_fail[e])) state[e] = 3;  // 3: collided
+                    if (e % W == m) gstore64(&V.g_key[e], kNoKey);  // read by every member before B3
+                }
+                if (tid == 0) sc[SC_OVERFLOW] = 0;
+                __syncthreads();
+                int base0 = size_a, base1 = size_b;
+                for (int e0 = 0; e0 < S; e0 += nt) {
+                    const int e = e0 + tid;
+                    const bool fr = e < S && state[e] == 1;
+                    const int t = fr ? side[e] : 0;
+                    int tot0, tot1;
+                    const int p0 = block_excl_scan((fr && t == 0) ? 1 : 0, scan, &tot0);
+                    const int p1 = block_excl_scan((fr && t == 1) ? 1 : 0, scan, &tot1);
+                    if (e < S) {
+                        int ni = -1;
+                        if (fr) {
+                            const int idx = t ? base1 + p1 : base0 + p0;
+                            if (idx < N) {
+                                ni = idx;
+                                if (e % W == m) {
+                                    float* T = t ? TB : TA;
+                                    for (int d = 0; d < dim; ++d)
+                                        stc<COH>(T + static_cast<size_t>(d) * ncap + idx, cand[e * dim + d]);
+                                    stc<COH>((t ? PB : PA) + idx, nn[e]);
+                                    stc<COH>((t ? RB : RA) + idx, FLT_MAX);
+                                }
+                            } else {
+                                sc[SC_OVERFLOW] = 1;
+                            }
+                        }
+                        newi[e] = ni;
+                    }
+                    base0 += tot0;
+                    base1 += tot1;
+                }
+                if (a.dd && m == 0) {
+                    for (int e = tid; e < S; e += nt) {
+                        if (state[e] == 3) hash_add(hk, hs, hg, a.hash_size, nn[e] * 2 + side[e], true);
+                        else if (newi[e] >= 0) hash_add(hk, hs, hg, a.hash_size, nn[e] * 2 + side[e], false);
+                    }
+                    __syncthreads();
+                    for (int h = tid; h < a.hash_size; h += nt) {
+                        const int key = hk[h];
+                        if (key < 0) continue;
+                        float* R = (key & 1) ? RB : RA;
+                        float rad = ldc<COH>(R + (key >> 1));
+                        for (int i = 0; i < hs[h]; ++i) radius_shrink(rad, a.dd_alpha, a.dd_radius, a.dd_min_radius);
+                        for (int i = 0; i < hg[h]; ++i) radius_grow(rad, a.dd_alpha);
+                        stc<COH>(R + (key >> 1), rad);
+                        hk[h] = -1; hs[h] = 0; hg[h] = 0;
+                    }
                 }
                 __syncthreads();
-                for (int h = tid; h < a.hash_size; h += nt) {
-                    const int key = hk[h];
-                    if (key < 0) continue;
-                    float* R = (key & 1) ? RB : RA;
-                    float rad = ldc<COH>(R + (key >> 1));
-                    for (int i = 0; i < hs[h]; ++i) radius_shrink(rad, a.dd_alpha, a.dd_radius, a.dd_min_radius);
-                    for (int i = 0; i < hg[h]; ++i) radius_grow(rad, a.dd_alpha);
-                    stc<COH>(R + (key >> 1), rad);
-                    hk[h] = -1; hs[h] = 0; hg[h] = 0;
+                if (tid == 0) {
+                    sc[SC_SIZE_A] = min(base0, N);
+                    sc[SC_SIZE_B] = min(base1, N);
+                    if (sc[SC_OVERFLOW]) sc[SC_STATUS] = PRRTC_STATUS_TREE_FULL;
                 }
             }
+            // No team barrier: the connect walk reads only nodes that existed at the start of the round
+            // (appends and radius updates are seen from the next round, after B7 / B8).
             __syncthreads();
-            if (tid == 0) {
-                sc[SC_SIZE_A] = min(base0, N);
-                sc[SC_SIZE_B] = min(base1, N);
-                if (sc[SC_OVERFLOW]) sc[SC_STATUS] = PRRTC_STATUS_TREE_FULL;
+            mark(2);
+            if (sc[SC_STATUS] != 0) {
+                ended = true;
+                break;
             }
-        }
-        // No team barrier: the connect walk reads only nodes that existed at the start of the round
-        // (appends and radius updates are seen from the next round, after B7 / B8).
-        __syncthreads();
-        mark(2);
-        if (sc[SC_STATUS] != 0) break;
-        if (W == 1) connect_nn(false, size_a, size_b);
-        for (int e = tid; e < S; e += nt) {
-            if (newi[e] < 0) continue;
-            const float* O = side[e] ? TA : TB;
-            const unsigned long long key = gload64(&V.g_okey[e]);
-            const int oi = static_cast<int>(key & 0xffffffffull);
-            onn[e] = oi;
-            float tgt[MAXD];  // the other tree's nearest node, loaded together
-#pragma unroll
-            for (int d = 0; d < MAXD; ++d) tgt[d] = ldc<COH>(O + static_cast<size_t>(min(d, dim - 1)) * ncap + oi);
-            const float cd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
-            int n_ext = 0;
-            if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
-            next[e] = n_ext;
-#pragma unroll
-            for (int d = 0; d < MAXD; ++d)
-                if (d < dim)
-                    vec[e * dim + d] = (n_ext > 0) ? (tgt[d] - cand[e * dim + d]) / static_cast<float>(n_ext) : 0.0f;
-        }
-        __syncthreads();
-        // ---- phase 5: connect walks.  Work unit = ku consecutive waypoints of sub-edge j of one
-        // explorer's walk; a collision at sub-edge j lowers the explorer's first failing sub-edge
-        // g_cfail[e] (an order-independent minimum), and units past it are skipped.
-        {
-            const int nl = compact(S, [&](int e) { return newi[e] >= 0; }, list, scan);
-            int nsub = 0;
-            for (int k0 = 0; k0 < nl; k0 += nt) {
-                const int k = k0 + tid;
-                int tot;
-                block_excl_scan(k < nl ? next[list[k]] : 0, scan, &tot);
-                nsub += tot;
+            if (W == 1) connect_nn(false, size_a, size_b);
+            for (int e = tid; e < S; e += nt) {
+                if (newi[e] < 0) continue;
+                const float* O = side[e] ? TA : TB;
+                const unsigned long long key = gload64(&V.g_okey[e]);
+                const int oi = static_cast<int>(key & 0xffffffffull);
+                onn[e] = oi;
+                float tgt[MAXD];  // the other tree's nearest node, loaded together
+    #pragma unroll
+                for (int d = 0; d < MAXD; ++d) tgt[d] = ldc<COH>(O + static_cast<size_t>(min(d, dim - 1)) * ncap + oi);
+                const float cd = __builtin_sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));
+                int n_ext = 0;
+                if (a.step_size > 1e-8f && cd > 1e-8f) n_ext = static_cast<int>(ceilf(cd / a.step_size));
+                next[e] = n_ext;
+    #pragma unroll
+                for (int d = 0; d < MAXD; ++d)
+                    if (d < dim)
+                        vec[e * dim + d] = (n_ext > 0) ? (tgt[d] - cand[e * dim + d]) / static_cast<float>(n_ext) : 0.0f;
             }
-            const int ku = unit_size(nsub, G, W * nwaves, fb);
-            const int cps = (G + ku - 1) / ku;
-            // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers in
-            // aux0).  Otherwise units are claimed dynamically, and a collision at sub-edge j lowers
-            // the explorer's cap to j, so units past the first failure stop being handed out.
-            const bool deal = static_cast<long long>(nsub) * G < W * nwaves;
-            int total = 0;
-            if (deal) {
+            __syncthreads();
+            }
+            int nl, ku, cps, total = 0;
+            bool deal;
+            if (kind == 0) {
+                nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
+                ku = unit_size(nl, G, W * nwaves, fb);
+                cps = (G + ku - 1) / ku;
+                // Fewer waypoints than waves: a fixed round-robin deal of one-waypoint units (no
+                // claiming; nothing is left to skip).  Otherwise units are claimed dynamically
+                // (claim counter per explorer), which balances the load and skips edges known to
+                // collide.
+                deal = static_cast<long long>(nl) * G < W * nwaves;
+            } else {
+                nl = compact(S, [&](int e) { return newi[e] >= 0; }, list, scan);
+                int nsub = 0;
                 for (int k0 = 0; k0 < nl; k0 += nt) {
                     const int k = k0 + tid;
                     int tot;
-                    const int pre = block_excl_scan(k < nl ? next[list[k]] * cps : 0, scan, &tot);
-                    if (k < nl) aux0[k] = total + pre;
-                    total += tot;
+                    block_excl_scan(k < nl ? next[list[k]] : 0, scan, &tot);
+                    nsub += tot;
                 }
-                __syncthreads();
+                ku = unit_size(nsub, G, W * nwaves, fb);
+                cps = (G + ku - 1) / ku;
+                // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers
+                // in aux0).  Otherwise units are claimed dynamically, and a collision at sub-edge j
+                // lowers the explorer's cap to j, so units past the first failure stop being
+                // handed out.
+                deal = static_cast<long long>(nsub) * G < W * nwaves;
+                if (deal) {
+                    for (int k0 = 0; k0 < nl; k0 += nt) {
+                        const int k = k0 + tid;
+                        int tot;
+                        const int pre = block_excl_scan(k < nl ? next[list[k]] * cps : 0, scan, &tot);
+                        if (k < nl) aux0[k] = total + pre;
+                        total += tot;
+                    }
+                    __syncthreads();
+                }
             }
             int us = m * nwaves + wid;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
                 int e = -1, jj = 0, k0 = 0;
-                if (deal) {
-                    if (us >= total) break;
-                    int lo = 0, hi = nl - 1;  // the last walker whose offset is <= us
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (aux0[mid] <= us) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    e = list[lo];
-                    const int rr = us - aux0[lo];
-                    jj = rr / cps;
-                    k0 = (rr - jj * cps) * ku;
-                    us += W * nwaves;
-                } else {
-                    for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
-                        bool ok = false;
-                        int ex = 0;
-                        if (b0 + lane < S) {
-                            ex = start + b0 + lane;
-                            if (ex >= S) ex -= S;
-                            if (newi[ex] >= 0) {
-                                const int cap = min(next[ex], gload(&V.g_cfail[ex])) * cps;
-                                ok = gload(&V.g_claim[ex]) < cap;
+                if (kind == 0) {
+                    if (deal) {
+                        if (us >= nl * cps) break;
+                        e = list[us / cps];
+                        k0 = (us - (us / cps) * cps) * ku;
+                        us += W * nwaves;
+                        if (uniform(gload(&V.g_fail[e]))) continue;
+                    } else {
+                        for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
+                            bool ok = false;
+                            int ex = 0;
+                            if (b0 + lane < S) {
+                                ex = start + b0 + lane;
+                                if (ex >= S) ex -= S;
+                                ok = state[ex] == 1 && gload(&V.g_fail[ex]) == 0 && gload(&V.g_eclaim[ex]) < cps;
                             }
+                            const unsigned long long mask = __ballot(ok);
+                            if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
                         }
-                        const unsigned long long mask = __ballot(ok);
-                        if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                        e = uniform(e);
+                        if (e < 0) break;
+                        start = e;
+                        int u = 0;
+                        if (lane == 0)
+                            u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_eclaim[e], 1, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                        u = uniform(__shfl(u, 0, kWave));
+                        if (u >= cps) continue;  // another wave took the last unit
+                        k0 = u * ku;
                     }
-                    e = uniform(e);
-                    if (e < 0) break;
-                    start = e;
-                    int u = 0;
-                    if (lane == 0)
-                        u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_claim[e], 1, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                    u = uniform(__shfl(u, 0, kWave));
-                    jj = u / cps;
-                    k0 = (u - jj * cps) * ku;
-                    if (jj >= next[e]) continue;  // past the end of the walk
+                } else {
+                    if (deal) {
+                        if (us >= total) break;
+                        int lo = 0, hi = nl - 1;  // the last walker whose offset is <= us
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (aux0[mid] <= us) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        e = list[lo];
+                        const int rr = us - aux0[lo];
+                        jj = rr / cps;
+                        k0 = (rr - jj * cps) * ku;
+                        us += W * nwaves;
+                    } else {
+                        for (int b0 = 0; b0 < S && e < 0; b0 += kWave) {
+                            bool ok = false;
+                            int ex = 0;
+                            if (b0 + lane < S) {
+                                ex = start + b0 + lane;
+                                if (ex >= S) ex -= S;
+                                if (newi[ex] >= 0) {
+                                    const int cap = min(next[ex], gload(&V.g_cfail[ex])) * cps;
+                                    ok = gload(&V.g_claim[ex]) < cap;
+                                }
+                            }
+                            const unsigned long long mask = __ballot(ok);
+                            if (mask) e = __shfl(ex, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+                        }
+                        e = uniform(e);
+                        if (e < 0) break;
+                        start = e;
+                        int u = 0;
+                        if (lane == 0)
+                            u = __hip_atomic_fetch_add((PRRTC_GLOBAL int*)&V.g_claim[e], 1, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                        u = uniform(__shfl(u, 0, kWave));
+                        jj = u / cps;
+                        k0 = (u - jj * cps) * ku;
+                        if (jj >= next[e]) continue;  // past the end of the walk
+                    }
+                    if (jj >= uniform(gload(&V.g_cfail[e]))) continue;  // past the first failure
                 }
-                if (jj >= uniform(gload(&V.g_cfail[e]))) continue;  // past the first failure
                 const int nk = min(ku, G - k0);
                 if (lane < nk) {
                     const float tf = static_cast<float>(k0 + lane + 1) / static_cast<float>(G);
                     for (int d = 0; d < dim; ++d) {
-                        float c = cand[e * dim + d];
-                        const float v = vec[e * dim + d];
-                        for (int q = 0; q < jj; ++q) c += v;
-                        L.q[d * fbp] = c + tf * v;
+                        if (kind == 0) {
+                            const float base = vec[e * dim + d];  // the nearest node (phase 1c)
+                            L.q[d * fbp] = base + tf * (cand[e * dim + d] - base);
+                        } else {
+                            float c = cand[e * dim + d];
+                            const float v = vec[e * dim + d];
+                            for (int q = 0; q < jj; ++q) c += v;
+                            L.q[d * fbp] = c + tf * v;
+                        }
                     }
                     if (M.enabled) fk_lane(M, L);
                 }
                 wave_sync();
-                for (int c = 0; c < nk; ++c) {
-                    // (the cap was read just before the unit; re-read from the second waypoint)
-                    if (c > 0 && jj >= uniform(gload(&V.g_cfail[e]))) break;
-                    if (lane == 0) ++evals;
-                    if (wave_in_collision(M, WA.T + c, fbp, flops)) {
-                        if (lane == 0) gmin(&V.g_cfail[e], jj);
+                // Sub-units of up to kUnitMax consecutive waypoints share one coarse pass
+                // (wave_unit_check); a sub-unit whose candidate list overflows is checked one
+                // waypoint at a time.  Only whether some waypoint collides is recorded.
+                for (int c0 = 0; c0 < nk; c0 += kUnitMax) {
+                    // (the flag / cap was read when the unit was taken; re-read from the second
+                    // sub-unit)
+                    if (c0 > 0 && (kind == 0 ? uniform(gload(&V.g_fail[e])) != 0 : jj >= uniform(gload(&V.g_cfail[e]))))
+                        break;
+                    const int nu = min(kUnitMax, nk - c0);
+                    int v = (nu > 1) ? wave_unit_in_collision(M, WA.T + c0, fbp, nu, flops) : -1;
+                    if (v < 0) {
+                        v = 0;
+                        for (int c = 0; c < nu && v == 0; ++c)
+                            if (wave_in_collision(M, WA.T + c0 + c, fbp, flops)) v = c + 1;
+                    }
+                    if (lane == 0) evals += (v > 0) ? v : nu;
+                    if (v > 0) {
+                        if (lane == 0) {
+                            if (kind == 0) gstore(&V.g_fail[e], 1);
+                            else gmin(&V.g_cfail[e], jj);
+                        }
                         break;
                     }
                 }
                 wave_sync();
             }
         }
+        if (ended) break;
         team_barrier(team);  // B7: every connect walk's length is in global memory
         mark(4);
         // ---- phase 6: connect appends (explorer-major); first success or overflow decides
@@ -1282,7 +1295,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
             PlanArgs t = a;
             t.ml = model_layout(robot->n_joints, robot->n_robot_spheres, world->n_self_pairs, world->n_world_spheres,
                                 world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt / kWave,
-                                fbp);
+                                fbp, kUnitCap);
             const size_t l = planner_lds(t.ml, S, dim, t);
             if (l > static_cast<size_t>(di.max_lds)) continue;
             const int occ = small ? occupancy<8>(nt, l) : occupancy<16>(nt, l);
