@@ -448,7 +448,7 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
 constexpr float kCullPad = 1e-3f;
 constexpr int kCullQ = 2 * kWave;     // refinement queue (obstacle * 64 + group) per wave
 constexpr int kUnitCap = 2 * kWave;   // candidate pairs of a sub-unit (wave_unit_check)
-constexpr int kUnitMax = 4;           // configurations per sub-unit
+constexpr int kUnitMax = 16;          // configurations per sub-unit (one FK batch, kFB)
 
 // Per-wave cull area: robot spheres, group spheres, refinement queue, and ucap candidate slots
 // (kUnitCap where wave_unit_check runs, else 0).
@@ -687,11 +687,12 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
 // list overflowed (the caller then checks them one by one).  Needs a cull area with kUnitCap
 // candidate slots.  Consecutive waypoints of one edge are close, so one coarse pass serves all
 // of them: group g's unit sphere has centre C_g (midpoint of its nu transformed centres) and
-// radius R_g + cut + pad + max_c |p_c - C_g| (wave_culled's coarse limit plus the spread).  The
-// coarse tests are 1-Lipschitz in the centre, so a pair the unit sphere clears is cleared by
-// every configuration's own group sphere (triangle inequality); the pairs it cannot clear form a
-// candidate list, and each configuration then runs wave_culled's coarse test on the candidates
-// only, its fine tests and the self pairs.  Verdicts are wave_culled's, one by one.
+// radius R_g + cut + pad + the half-diagonal of the box around them (wave_culled's coarse limit
+// plus a bound on max_c |p_c - C_g|).  The coarse tests are 1-Lipschitz in the centre, so a pair
+// the unit sphere clears is cleared by every configuration's own group sphere (triangle
+// inequality); the pairs it cannot clear form a candidate list, and each configuration then runs
+// wave_culled's coarse test on the candidates only, its fine tests and the self pairs.  Verdicts
+// are wave_culled's, one by one.
 template <bool MARGIN>
 __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -712,31 +713,20 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
     if (lane < ng) {
         const int g = lane;
         const float4 c = M.gcoarse[g];
-        float px[kUnitMax], py[kUnitMax], pz[kUnitMax];
         float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
-#pragma unroll
-        for (int k = 0; k < kUnitMax; ++k) {
-            px[k] = 0.0f; py[k] = 0.0f; pz[k] = 0.0f;
-            if (k < nu) {
-                Mat3 R;
-                float tx, ty, tz;
-                frame_from(T + k, ts, M.gframe[g], R, tx, ty, tz);
-                const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
-                px[k] = p.x; py[k] = p.y; pz[k] = p.z;
-                lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
-                hx = fmaxf(hx, p.x); hy = fmaxf(hy, p.y); hz = fmaxf(hz, p.z);
-            }
+        for (int k = 0; k < nu; ++k) {
+            Mat3 R;
+            float tx, ty, tz;
+            frame_from(T + k, ts, M.gframe[g], R, tx, ty, tz);
+            const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
+            lx = fminf(lx, p.x); ly = fminf(ly, p.y); lz = fminf(lz, p.z);
+            hx = fmaxf(hx, p.x); hy = fmaxf(hy, p.y); hz = fmaxf(hz, p.z);
         }
+        // every centre lies in the box [l, h]: within its half-diagonal of the midpoint
         const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
-        float spread = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kUnitMax; ++k) {
-            if (k < nu) {
-                const float dx = px[k] - cx, dy = py[k] - cy, dz = pz[k] - cz;
-                spread = fmaxf(spread, __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))));
-            }
-        }
-        // the spread rounded up (relative 1e-6); the coarse pad covers the rest of the rounding
+        const float ex = hx - lx, ey = hy - ly, ez = hz - lz;
+        const float spread = 0.5f * __builtin_sqrtf(__builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)));
+        // the spread rounded up (relative 1e-6); kCullPad covers the midpoint's rounding
         CU[g] = make_float4(cx, cy, cz, (c.w + cut + kCullPad) + spread * 1.000001f);
     }
     flops += static_cast<unsigned long long>(ng) * nu * kFlopsXf;

@@ -294,10 +294,12 @@ __device__ __forceinline__ int unit_size(int edges, int G, int waves, int fb) {
 constexpr int kSyncLine = 64;     // words per line
 constexpr int kSyncGroups = 16;   // group counters of a large team
 constexpr int kGroupMinW = 128;   // smallest team that arrives in groups
-__host__ __device__ inline int sync_group_count(int W) {  // a divisor of W (the grid's W need not be 2^k)
-    int G = W >= 2 * kGroupMinW ? kSyncGroups : (W >= kGroupMinW ? kSyncGroups / 2 : 1);
-    while (G > 1 && W % G != 0) G >>= 1;
-    return G;
+__host__ __device__ inline int sync_group_count(int W) {
+    return W >= 2 * kGroupMinW ? kSyncGroups : (W >= kGroupMinW ? kSyncGroups / 2 : 1);
+}
+// Members of group g (members m with m % G == g): W / G, one more for the first W % G groups.
+__device__ __forceinline__ unsigned group_size(int W, int G, int g) {
+    return static_cast<unsigned>(W / G + (g < W % G ? 1 : 0));
 }
 
 // Team of W work-groups working on one query.
@@ -318,7 +320,7 @@ __device__ __forceinline__ void drain_memory() { asm volatile("s_waitcnt vmcnt(0
 // that each thread's accesses have completed before its work-group counts in: relaxed counter
 // and generation words, no L2 write-back or invalidate (a team may span XCDs).  A large team
 // arrives in G groups: the last arrival of a group resets the group counter, waits for that
-// store and counts in at the top counter, so no counter sees more than max(W / G, G) arrivals
+// store and counts in at the top counter, so no counter sees more than max(ceil(W / G), G) arrivals
 // (atomics on one address are serialised).  The last arrival at the top resets it, waits for
 // that store, then opens the next generation; a member can only reach the next barrier after
 // that, when every reset has completed.  Waiting is bounded (seconds), so a broken team raises
@@ -338,7 +340,7 @@ __device__ __forceinline__ void team_barrier(Team& t) {
         if (threadIdx.x == 0) {
             unsigned* gen = t.sync + kSyncLine;
             bool last = true;
-            if (t.G > 1) last = count_in(t.grp + (t.m % t.G) * kSyncLine, static_cast<unsigned>(t.W / t.G));
+            if (t.G > 1) last = count_in(t.grp + (t.m % t.G) * kSyncLine, group_size(t.W, t.G, t.m % t.G));
             if (last) last = count_in(t.sync, static_cast<unsigned>(t.G > 1 ? t.G : t.W));
             if (last) {
                 __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
