@@ -449,6 +449,7 @@ constexpr float kCullPad = 1e-3f;
 constexpr int kCullQ = 2 * kWave;     // refinement queue (obstacle * 64 + group) per wave
 constexpr int kUnitCap = 2 * kWave;   // candidate pairs of a sub-unit (wave_unit_check)
 constexpr int kUnitMax = 16;          // configurations per sub-unit (one FK batch, kFB)
+static_assert(kUnitMax <= 16, "wave_unit_check packs the configuration into 4 bits of a queue entry");
 
 // Per-wave cull area: robot spheres, group spheres, refinement queue, and ucap candidate slots
 // (kUnitCap where wave_unit_check runs, else 0).
@@ -690,15 +691,16 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
 // radius R_g + cut + pad + the half-diagonal of the box around them (wave_culled's coarse limit
 // plus a bound on max_c |p_c - C_g|).  The coarse tests are 1-Lipschitz in the centre, so a pair
 // the unit sphere clears is cleared by every configuration's own group sphere (triangle
-// inequality); the pairs it cannot clear form a candidate list, and each configuration then runs
-// wave_culled's coarse test on the candidates only, its fine tests and the self pairs.  Verdicts
-// are wave_culled's, one by one.
+// inequality); the pairs it cannot clear form a candidate list.  The candidates of all
+// configurations are then tested together (wave_culled's coarse and fine tests, 64 pairs per wave
+// step), and the self pairs of the configurations before the first world hit one by one.
+// Verdicts are wave_culled's.
 template <bool MARGIN>
 __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
     char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
     float4* FP = reinterpret_cast<float4*>(area);          // robot spheres (per configuration)
-    float4* CW = FP + M.n_spheres;                          // group spheres (per configuration)
+    float4* CW = FP + M.n_spheres;                          // (wave_culled's group spheres)
     int* Q = reinterpret_cast<int*>(CW + M.n_joints);       // refinement queue
     int* LU = Q + kCullQ;                                   // candidate pairs of the sub-unit
     float4* CU = FP;  // unit spheres (n_groups <= n_spheres); free until the first transform below
@@ -781,82 +783,93 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         }
     }
     wave_sync();
-    // ---- each configuration: its spheres and group spheres, the candidates, the self pairs
-    for (int u = 0; u < nu; ++u) {
-        const float* Tu = T + u;
-        for (int s0 = 0; s0 < nrs; s0 += kWave) {
-            const int s = s0 + lane;
-            bool act = false;
-            if (s < nrs) {
-                const int js = M.sphere_js[s];
-                float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (js >= 0) {
-                    Mat3 R;
-                    float tx, ty, tz;
-                    frame_from(Tu, ts, js, R, tx, ty, tz);
-                    const float4 sp = M.spheres[s];
-                    const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
-                    o = make_float4(p.x, p.y, p.z, sp.w);
-                    act = true;
-                }
-                FP[s] = o;
-            }
-            flops += static_cast<unsigned long long>(__popcll(__ballot(act))) * kFlopsXf;
-        }
-        if (lane < ng) {
-            const int g = lane;
+    // ---- world: every (configuration, candidate) pair at once, lanes over pairs.  The coarse
+    // test uses the configuration's own group sphere (wave_culled's); the fine tests of a queued
+    // (configuration, obstacle, group) triple transform the group's member spheres with their
+    // frame exactly as wave_check does.  hm: bit u set if configuration u hits the world.
+    unsigned hm = 0;
+    const int nc = nu * nl;
+    for (int k0 = 0; k0 < nc; k0 += kWave) {
+        const int k = k0 + lane;
+        bool dirty = false;
+        int entry = 0;
+        if (k < nc) {
+            const int u = k / nl;
+            const int en = LU[k - u * nl];
+            const int o = en >> 6, g = en & 63;
             const float4 c = M.gcoarse[g];
             Mat3 R;
             float tx, ty, tz;
-            frame_from(Tu, ts, M.gframe[g], R, tx, ty, tz);
+            frame_from(T + u, ts, M.gframe[g], R, tx, ty, tz);
             const float3 p = mat_apply(R, tx, ty, tz, c.x, c.y, c.z);
-            CW[g] = make_float4(p.x, p.y, p.z, c.w + cut + kCullPad);
+            const float lim = c.w + cut + kCullPad;
+            dirty = !coarse_clear(M, o, make_float4(p.x, p.y, p.z, lim), lim);
+            lf += kFlopsXf + ((o < nwh) ? kFlopsSH : (o < nwh + nwb) ? kFlopsSB : (o < no) ? kFlopsSC : kFlopsSS);
+            entry = (o << 10) | (u << 6) | g;
         }
-        // per configuration, as wave_in_collision counts it
-        flops += static_cast<unsigned long long>(ng) * kFlopsXf + static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
+        const unsigned long long mask = __ballot(dirty);
+        if (dirty) Q[static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
+        const int qn = static_cast<int>(__popcll(mask));  // <= kWave <= kCullQ
         wave_sync();
-        bool hit = false;
-        for (int k0 = 0; k0 < nl && !hit; k0 += kWave) {
-            bool dirty = false;
-            int entry = 0;
-            if (k0 + lane < nl) {
-                entry = LU[k0 + lane];
-                const int o = entry >> 6, g = entry & 63;
-                const float4 c = CW[g];
-                dirty = !coarse_clear(M, o, c, c.w);
-                lf += (o < nwh) ? kFlopsSH : (o < nwh + nwb) ? kFlopsSB : (o < no) ? kFlopsSC : kFlopsSS;
-            }
-            const unsigned long long mask = __ballot(dirty);
-            if (dirty) Q[static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
-            const int qn = static_cast<int>(__popcll(mask));
-            wave_sync();
-            bool h = false;
-            for (int e = lane; e < qn && !h; e += kWave) {
-                const int o = Q[e] >> 6, g = Q[e] & 63;
-                for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
-                    if (fine_test<MARGIN>(M, o, FP[M.gmem[m]], lf)) {
-                        h = true;
-                        break;
-                    }
+        for (int e = lane; e < qn; e += kWave) {
+            const int o = Q[e] >> 10, u = (Q[e] >> 6) & 15, g = Q[e] & 63;
+            Mat3 R;
+            float tx, ty, tz;
+            frame_from(T + u, ts, M.gframe[g], R, tx, ty, tz);
+            for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
+                const float4 sp = M.spheres[M.gmem[m]];
+                const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+                lf += kFlopsXf;
+                if (fine_test<MARGIN>(M, o, make_float4(p.x, p.y, p.z, sp.w), lf)) {
+                    hm |= 1u << u;
+                    break;
                 }
             }
-            hit = wave_any(h);
-            wave_sync();
-        }
-        if (hit) return done(u + 1);
-        for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
-            const int k = p0 + lane;
-            bool h = false;
-            if (k < M.n_pairs) {
-                const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
-                h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
-            }
-            flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
-            if (wave_any(h)) return done(u + 1);
         }
         wave_sync();
     }
-    return done(0);
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) hm |= static_cast<unsigned>(__shfl_xor(static_cast<int>(hm), o, kWave));
+    const int first = hm ? __builtin_ctz(hm) : nu;  // the first configuration with a world hit
+    flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints * (first < nu ? first + 1 : nu);
+    // ---- self pairs of the configurations before it, in order
+    if (M.n_pairs > 0) {
+        for (int u = 0; u < first; ++u) {
+            const float* Tu = T + u;
+            for (int s0 = 0; s0 < nrs; s0 += kWave) {
+                const int s = s0 + lane;
+                bool act = false;
+                if (s < nrs) {
+                    const int js = M.sphere_js[s];
+                    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (js >= 0) {
+                        Mat3 R;
+                        float tx, ty, tz;
+                        frame_from(Tu, ts, js, R, tx, ty, tz);
+                        const float4 sp = M.spheres[s];
+                        const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
+                        o = make_float4(p.x, p.y, p.z, sp.w);
+                        act = true;
+                    }
+                    FP[s] = o;
+                }
+                flops += static_cast<unsigned long long>(__popcll(__ballot(act))) * kFlopsXf;
+            }
+            wave_sync();
+            for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
+                const int k = p0 + lane;
+                bool h = false;
+                if (k < M.n_pairs) {
+                    const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
+                    h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+                }
+                flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
+                if (wave_any(h)) return done(u + 1);
+            }
+            wave_sync();
+        }
+    }
+    return done(first < nu ? first + 1 : 0);
 }
 
 // The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform; the

@@ -319,29 +319,33 @@ __device__ __forceinline__ void drain_memory() { asm volatile("s_waitcnt vmcnt(0
 // members share goes through agent-scope atomics (ldc/stc<true>, gload/gstore), so it suffices
 // that each thread's accesses have completed before its work-group counts in: relaxed counter
 // and generation words, no L2 write-back or invalidate (a team may span XCDs).  A large team
-// arrives in G groups: the last arrival of a group resets the group counter, waits for that
-// store and counts in at the top counter, so no counter sees more than max(ceil(W / G), G) arrivals
-// (atomics on one address are serialised).  The last arrival at the top resets it, waits for
-// that store, then opens the next generation; a member can only reach the next barrier after
-// that, when every reset has completed.  Waiting is bounded (seconds), so a broken team raises
-// the error flag instead of hanging.
-__device__ __forceinline__ bool count_in(unsigned* cnt, unsigned n) {
+// arrives in G groups: the last arrival of a group counts in at the top counter, so no counter
+// sees more than max(ceil(W / G), G) arrivals (atomics on one address are serialised).  The last
+// arrival at the top opens the next generation.  A counter of n arrivals per barrier is not reset
+// between barriers: arrival k * n (mod n == 0) is the last one of a barrier, since no member can
+// arrive at the next barrier before the generation opens, after every arrival of this one.  The
+// team's last barrier (`reset`, before its words go to another team shape) resets each counter
+// and waits for that store before counting in above it or opening the generation.  Waiting is
+// bounded (seconds), so a broken team raises the error flag instead of hanging.
+__device__ __forceinline__ bool count_in(unsigned* cnt, unsigned n, bool reset) {
     const unsigned prev = __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)cnt, 1u, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
-    if (prev != n - 1u) return false;
-    __hip_atomic_store((PRRTC_GLOBAL unsigned*)cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    drain_memory();
+    if ((prev + 1u) % n != 0u) return false;
+    if (reset) {
+        __hip_atomic_store((PRRTC_GLOBAL unsigned*)cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        drain_memory();
+    }
     return true;
 }
-__device__ __forceinline__ void team_barrier(Team& t) {
+__device__ __forceinline__ void team_barrier(Team& t, bool reset = false) {
     if (t.W > 1) {
         drain_memory();
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned* gen = t.sync + kSyncLine;
             bool last = true;
-            if (t.G > 1) last = count_in(t.grp + (t.m % t.G) * kSyncLine, group_size(t.W, t.G, t.m % t.G));
-            if (last) last = count_in(t.sync, static_cast<unsigned>(t.G > 1 ? t.G : t.W));
+            if (t.G > 1) last = count_in(t.grp + (t.m % t.G) * kSyncLine, group_size(t.W, t.G, t.m % t.G), reset);
+            if (last) last = count_in(t.sync, static_cast<unsigned>(t.G > 1 ? t.G : t.W), reset);
             if (last) {
                 __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
@@ -1153,7 +1157,9 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
                 const QueryView V = query_view(a, a.cont[cur * a.B + k]);
                 run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], budget, t0, 2,
                                 a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
-                team_barrier(team);  // members leave the query together before the next one
+                // members leave the query together before the next one; the team's last barrier
+                // resets its counters for the next epoch's team shapes
+                team_barrier(team, k + nteams >= ncont);
             }
         }
         grid_barrier(grid);
