@@ -345,14 +345,20 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
         }
     };
     if constexpr (MAXD <= 8) {
-        // Two nodes per step, i and i + nt (both loads issued together), scored against each
-        // query row read once.  A thread visits its nodes in increasing index order (i before
+        // Two nodes per step, i and i + nt, scored against each query row read once; the next
+        // step's nodes are loaded before this step is scored (indices clamped, so the loads past
+        // the end are harmless).  A thread visits its nodes in increasing index order (i before
         // i + nt), so on equal distances the lower index still wins.
+        float xa[MAXD], xb[MAXD];
+        if (begin + tid < end) {
+            load_node(begin + tid, xa);
+            load_node(begin + tid + nt, xb);
+        }
         for (int i = begin + tid; i < end; i += 2 * nt) {
             asm volatile("" : "+v"(qoff));
-            float xa[MAXD], xb[MAXD];
-            load_node(i, xa);
-            load_node(i + nt, xb);
+            float ya[MAXD], yb[MAXD];
+            load_node(i + 2 * nt, ya);
+            load_node(i + 3 * nt, yb);
             const bool hb = i + nt < end;
 #pragma unroll
             for (int c = 0; c < kEC; ++c) {
@@ -368,6 +374,8 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
                 if (da < bd[c]) { bd[c] = da; bi[c] = i; }
                 if (hb && db < bd[c]) { bd[c] = db; bi[c] = i + nt; }
             }
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) { xa[d] = ya[d]; xb[d] = yb[d]; }
         }
     } else {
         for (int i = begin + tid; i < end; i += nt) {
