@@ -305,3 +305,62 @@ def test_coarse_culling_never_drops_a_hit(name):
                 culled += int(cull.sum())
                 kept += int((~cull).sum())
     assert culled > kept  # the bound prunes most pairs
+
+
+@pytest.mark.parametrize("name", ["c2", "tree"])
+def test_unit_culling_never_drops_a_hit(name):
+    """The bound of the planner's sub-unit check, in float32: for up to 16 consecutive waypoints
+    of an edge, a frame's unit sphere (midpoint of the box around the waypoints' bounding-sphere
+    centres, radius R + cut + 1 mm + the box's half-diagonal x 1.000001) that is farther from an
+    obstacle than that radius + |r_o| proves that no sphere of the frame hits the obstacle (binary,
+    squared form) or comes within the margin at any of the waypoints.  Without the half-diagonal
+    the same sample drops hits, so the spread term is what the bound rests on."""
+    cc, dim = _robots()[name]
+    cc = dict(cc)
+    rng0 = np.random.default_rng(7)
+    cc["world_spheres"] = np.concatenate([rng0.uniform(-0.6, 0.6, (60, 3)), rng0.uniform(0.01, 0.1, (60, 1))],
+                                         1).astype(np.float32)
+    ctx = O.Context(cc, dim)
+    inv = np.asarray(cc["fk_topo_inv"])
+    n = np.asarray(cc["fk_parent_idx"]).shape[0]
+    groups = _frame_groups(cc, n)
+    ws = np.asarray(cc["world_spheres"], np.float32).reshape(-1, 4)
+    rad = np.asarray(cc["sphere_radius"], np.float32)
+    pad, margin = np.float32(1e-3), np.float32(0.02)
+    rng = np.random.default_rng(3)
+    culled = kept = dropped_without_spread = 0
+    for _ in range(60):
+        a = rng.uniform(-2.8, 2.8, size=dim).astype(np.float32)
+        d = rng.normal(size=dim).astype(np.float32)
+        b = (a + np.float32(rng.uniform(0.1, 0.8)) * d / np.float32(np.linalg.norm(d))).astype(np.float32)
+        nu = int(rng.integers(2, 17))
+        qs = [(a + np.float32((k + 1) / 16.0) * (b - a)).astype(np.float32) for k in range(nu)]
+        Ts = [O.fk(ctx, q) for q in qs]
+        Ps = [O.sphere_positions(ctx, q) for q in qs]
+        for j, mem, c, R in groups:
+            row = np.flatnonzero(inv == j)[0]
+            cws = np.stack([(O.quat_mat64(T[row, :4]).astype(np.float32) @ c + T[row, 4:]).astype(np.float32)
+                            for T in Ts])
+            lo, hi = cws.min(0), cws.max(0)
+            cu = (np.float32(0.5) * (lo + hi)).astype(np.float32)
+            e = (hi - lo).astype(np.float32)
+            spread = np.float32(0.5) * np.float32(np.sqrt(np.float32(e @ e)))
+            du = np.linalg.norm(ws[:, :3] - cu, axis=1).astype(np.float32)
+            for cut in (np.float32(0.0), margin):
+                lim = (R + cut + pad) + spread * np.float32(1.000001)
+                cull = du > lim + np.abs(ws[:, 3])
+                for o in np.flatnonzero(cull):
+                    for P in Ps:
+                        dd = np.linalg.norm(P[mem] - ws[o, :3], axis=1)
+                        assert np.all(dd > np.abs(rad[mem] + ws[o, 3])), (j, o)
+                        assert np.all(dd - rad[mem] - ws[o, 3] >= cut), (j, o)
+                culled += int(cull.sum())
+                kept += int((~cull).sum())
+                # the bound without the spread: count the pairs it would clear wrongly
+                bad = du > (R + cut + pad) + np.abs(ws[:, 3])
+                for o in np.flatnonzero(bad & ~cull):
+                    if any(np.any(np.linalg.norm(P[mem] - ws[o, :3], axis=1) - rad[mem] - ws[o, 3] < cut)
+                           for P in Ps):
+                        dropped_without_spread += 1
+    assert culled > kept
+    assert dropped_without_spread > 0
