@@ -770,16 +770,24 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         }
         flops += coarse_flops_of(M, total, ng, npo);
     }
+    // Sphere chunks: each lane tests its world sphere against every unit sphere (independent LDS
+    // broadcasts and distance tests, no per-group ballot) into a mask of groups, then the few set
+    // bits are pushed, one per lane per pass.
     for (int k0 = 0; k0 < full; k0 += kWave) {
         const float4 w = M.wsph[k0 + lane];
         const float rw = fabsf(w.w);
+        unsigned long long gm = 0;
         for (int sg = 0; sg < ng; ++sg) {
             const float4 c = CU[sg];
             const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
-            flops += static_cast<unsigned long long>(kWave) * kFlopsSS;
-            if (!push(!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw),
-                      ((no + k0 + lane) << 6) | sg))
-                return done(-1);
+            if (!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw)) gm |= 1ull << sg;
+        }
+        flops += static_cast<unsigned long long>(kWave) * ng * kFlopsSS;
+        while (wave_any(gm != 0)) {
+            const bool has = gm != 0;
+            const int sg = has ? static_cast<int>(__builtin_ctzll(gm)) : 0;
+            if (!push(has, ((no + k0 + lane) << 6) | sg)) return done(-1);
+            gm &= gm - 1ull;
         }
     }
     wave_sync();
