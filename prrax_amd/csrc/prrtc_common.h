@@ -344,21 +344,33 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
             xo[d] = (d < dim) ? v : 0.0f;
         }
     };
+    // Raw coordinates (dimensions d >= dim load row dim - 1; the caller zeroes them before use).
+    auto load_raw = [&](int i, float* xo) {
+        const int il = min(i, end - 1);
+#pragma unroll
+        for (int d = 0; d < MAXD; ++d) xo[d] = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
+    };
     if constexpr (MAXD <= 8) {
         // Two nodes per step, i and i + nt, scored against each query row read once; the next
         // step's nodes are loaded before this step is scored (indices clamped, so the loads past
-        // the end are harmless).  A thread visits its nodes in increasing index order (i before
-        // i + nt), so on equal distances the lower index still wins.
-        float xa[MAXD], xb[MAXD];
+        // the end are harmless), and kept raw until then, so that no wait for them comes before
+        // the scoring.  A thread visits its nodes in increasing index order (i before i + nt), so
+        // on equal distances the lower index still wins.
+        float ra[MAXD], rb[MAXD];
         if (begin + tid < end) {
-            load_node(begin + tid, xa);
-            load_node(begin + tid + nt, xb);
+            load_raw(begin + tid, ra);
+            load_raw(begin + tid + nt, rb);
         }
         for (int i = begin + tid; i < end; i += 2 * nt) {
             asm volatile("" : "+v"(qoff));
-            float ya[MAXD], yb[MAXD];
-            load_node(i + 2 * nt, ya);
-            load_node(i + 3 * nt, yb);
+            float xa[MAXD], xb[MAXD];
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) {
+                xa[d] = (d < dim) ? ra[d] : 0.0f;
+                xb[d] = (d < dim) ? rb[d] : 0.0f;
+            }
+            load_raw(i + 2 * nt, ra);
+            load_raw(i + 3 * nt, rb);
             const bool hb = i + nt < end;
 #pragma unroll
             for (int c = 0; c < kEC; ++c) {
@@ -374,8 +386,6 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
                 if (da < bd[c]) { bd[c] = da; bi[c] = i; }
                 if (hb && db < bd[c]) { bd[c] = db; bi[c] = i + nt; }
             }
-#pragma unroll
-            for (int d = 0; d < MAXD; ++d) { xa[d] = ya[d]; xb[d] = yb[d]; }
         }
     } else {
         for (int i = begin + tid; i < end; i += nt) {
