@@ -321,9 +321,13 @@ __device__ __forceinline__ bool lex_less(float da, int ia, float db, int ib) {
 // all outstanding memory traffic.
 __device__ __forceinline__ PRRTC_LDS float* lds_f(float* p) { return (PRRTC_LDS float*)(p); }
 __device__ __forceinline__ PRRTC_LDS int* lds_i(int* p) { return (PRRTC_LDS int*)(p); }
+// COH: nodes at indices >= coh_from may have been written by other work-groups since the last
+// grid-wide fence, so they are read with coherent loads; older nodes never change and are read
+// through the caches (a wave reads coherently if any of its nodes needs it).
 template <int MAXD, bool COH = false>
 __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int end, int dim,
-                        const PRRTC_LDS float* qsh, PRRTC_LDS float* red_d, PRRTC_LDS int* red_i) {
+                        const PRRTC_LDS float* qsh, PRRTC_LDS float* red_d, PRRTC_LDS int* red_i,
+                        int coh_from = 0) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int nw = (nt + kWave - 1) / kWave;
     float bd[kEC];
@@ -338,17 +342,24 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
     int qoff = 0;
     auto load_node = [&](int i, float* xo) {
         const int il = min(i, end - 1);
+        const bool coh = COH && __ballot(il >= coh_from) != 0ull;
 #pragma unroll
         for (int d = 0; d < MAXD; ++d) {
-            const float v = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
+            const float* p = tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il;
+            const float v = coh ? ldc<true>(p) : ldc<false>(p);
             xo[d] = (d < dim) ? v : 0.0f;
         }
     };
     // Raw coordinates (dimensions d >= dim load row dim - 1; the caller zeroes them before use).
     auto load_raw = [&](int i, float* xo) {
         const int il = min(i, end - 1);
+        if (COH && __ballot(il >= coh_from) != 0ull) {
 #pragma unroll
-        for (int d = 0; d < MAXD; ++d) xo[d] = ldc<COH>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
+            for (int d = 0; d < MAXD; ++d) xo[d] = ldc<true>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
+        } else {
+#pragma unroll
+            for (int d = 0; d < MAXD; ++d) xo[d] = ldc<false>(tree + static_cast<size_t>(min(d, dim - 1)) * ncap + il);
+        }
     };
     if constexpr (MAXD <= 8) {
         // Two nodes per step, i and i + nt, scored against each query row read once; the next

@@ -454,6 +454,9 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     }
     __syncthreads();
 
+    // Nodes that existed when this run began were written before the last grid-wide fence (or
+    // kernel boundary) and never change: NN scans read them through the caches.
+    const int coh_a = sc[SC_SIZE_A], coh_b = sc[SC_SIZE_B];
     unsigned long long evals = 0, flops = 0;  // lane 0 of each wave counts
     // Per-phase wall time (member 0, thread 0): up to B1, B3 (extend checks and connect NN), the
     // commit, (slot 3 unused), B7, B8.
@@ -487,15 +490,22 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const int c0 = t ? n0 + (k - nt0) * kEC : k * kEC;
             const int nc = min(kEC, (t ? n0 + n1 : n0) - c0);
             const int sz = t ? sz1 : sz0;
-            const int lo = static_cast<int>((static_cast<long long>(sz) * part) / parts);
-            const int hi = static_cast<int>((static_cast<long long>(sz) * (part + 1)) / parts);
+            // node range [lo, hi) = [sz * part / parts, sz * (part + 1) / parts), in 32 bits when
+            // the products fit (a 64-bit division is a long instruction sequence)
+            const unsigned long long p1 = static_cast<unsigned long long>(sz) * (part + 1);
+            const unsigned long long p0 = p1 - static_cast<unsigned long long>(sz);
+            const int lo = (p1 >> 32) ? static_cast<int>(p0 / parts)
+                                      : static_cast<int>(static_cast<unsigned>(p0) / static_cast<unsigned>(parts));
+            const int hi = (p1 >> 32) ? static_cast<int>(p1 / parts)
+                                      : static_cast<int>(static_cast<unsigned>(p1) / static_cast<unsigned>(parts));
             if (lo >= hi) continue;
             for (int q = tid; q < kEC * MAXD; q += nt) {
                 const int c = q / MAXD, d = q - (q / MAXD) * MAXD;
                 qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qpt(list[c0 + c], d) : 0.0f;
             }
             __syncthreads();
-            nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i));
+            nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
+                               t ? coh_b : coh_a);
             if (tid < nc) gmin64(&keys[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
             __syncthreads();
         }
