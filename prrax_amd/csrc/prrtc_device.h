@@ -616,12 +616,12 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     };
     // Pair steps (halfspaces, boxes, capsules, then the spheres past the last whole chunk): pair
     // p = o * ng + g (obstacle-major), lane l takes p0 + l, and (o, g) advance by kWave pairs per
-    // step without a division (ng <= 64).  Sphere steps (the many): lanes over a chunk of 64 world
-    // spheres, each loaded once, one group per step: one distance test per lane, no per-lane type
-    // dispatch (coarse_clear's sphere test).  Steps fill the queue up to flush_at; one refinement
-    // site serves both.
+    // step without a division (ng <= 64).  Sphere chunks (the many): lanes over a chunk of 64
+    // world spheres, each lane testing its sphere against every group sphere (coarse_clear's
+    // sphere test; independent LDS broadcasts, no per-group ballot) into a mask of groups, whose
+    // set bits are then queued one per lane per step.  Steps fill the queue up to flush_at; one
+    // refinement site serves both.
     const int psteps = (total + kWave - 1) / kWave;
-    const int ssteps = (full / kWave) * ng;
     int o = 0, g = 0, dq = 0, dr = 0;
     if (total > 0) {
         dq = kWave / ng;
@@ -629,9 +629,9 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         o = lane / ng;
         g = lane - (lane / ng) * ng;
     }
-    int pstep = 0, sstep = 0, sg = 0, k0 = 0;
-    float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float rw = 0.0f;
+    int pstep = 0, k0 = 0, chunks = 0;  // chunks: sphere chunks whose masks are computed
+    unsigned long long gm = 0;         // this lane's groups still to queue (chunk k0)
+    bool loaded = false;
     for (;;) {
         for (; pstep < psteps && qn < flush_at; ++pstep) {
             bool dirty = false;
@@ -648,28 +648,39 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
             push(dirty, entry);
         }
         if (pstep >= psteps) {
-            for (; sstep < ssteps && qn < flush_at; ++sstep) {
-                if (sg == 0) {
-                    w = M.wsph[k0 + lane];
-                    rw = fabsf(w.w);
+            while (k0 < full && qn < flush_at) {
+                if (!loaded) {
+                    const float4 w = M.wsph[k0 + lane];
+                    const float rw = fabsf(w.w);
+                    gm = 0;
+                    for (int sg = 0; sg < ng; ++sg) {
+                        const float4 c = CW[sg];
+                        const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+                        if (!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw)) gm |= 1ull << sg;
+                    }
+                    loaded = true;
+                    ++chunks;
                 }
-                const float4 c = CW[sg];
-                const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
-                push(!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw),
-                     ((no + k0 + lane) << 6) | sg);
-                if (++sg == ng) { sg = 0; k0 += kWave; }
+                if (!wave_any(gm != 0)) {
+                    k0 += kWave;
+                    loaded = false;
+                    continue;
+                }
+                const bool has = gm != 0;
+                push(has, ((no + k0 + lane) << 6) | (has ? static_cast<int>(__builtin_ctzll(gm)) : 0));
+                gm &= gm - 1ull;
             }
         }
-        const bool all = pstep >= psteps && sstep >= ssteps;
+        const bool all = pstep >= psteps && k0 >= full;
         if (qn > 0 && refine()) {
             flops += coarse_flops(min(pstep * kWave, total)) +
-                     static_cast<unsigned long long>(sstep) * kWave * kFlopsSS;
+                     static_cast<unsigned long long>(chunks) * kWave * ng * kFlopsSS;
             return done(true);
         }
         if (all) break;
         flush_at = min(2 * flush_at, kWave);
     }
-    flops += coarse_flops(total) + static_cast<unsigned long long>(ssteps) * kWave * kFlopsSS;
+    flops += coarse_flops(total) + static_cast<unsigned long long>(full) * ng * kFlopsSS;
     for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
         const int k = p0 + lane;
         bool h = false;
