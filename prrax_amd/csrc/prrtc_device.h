@@ -527,12 +527,11 @@ __device__ __forceinline__ unsigned long long pairs_in(long long p0, long long p
     return b > a ? static_cast<unsigned long long>(b - a) : 0ull;
 }
 
-// Flops of the first p1 pairs of the pair steps (halfspaces, boxes, capsules, then npo - no
-// spheres; npo pair-step obstacles, ng groups).
-__device__ __forceinline__ unsigned long long coarse_flops_of(const Model& M, int p1, int ng, int npo) {
+// Flops of the first p1 pairs of the pair steps (halfspaces, boxes, capsules; ng groups).
+__device__ __forceinline__ unsigned long long coarse_flops_of(const Model& M, int p1, int ng) {
     const int nwh = M.n_wh, nwb = M.n_wb, no = M.n_wh + M.n_wb + M.n_wc;
     return pairs_in(0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
-           pairs_in(0, p1, ng, nwh + nwb, no) * kFlopsSC + pairs_in(0, p1, ng, no, npo) * kFlopsSS;
+           pairs_in(0, p1, ng, nwh + nwb, no) * kFlopsSC;
 }
 
 template <bool MARGIN>
@@ -580,14 +579,7 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     wave_sync();
     const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
     const int no = nwh + nwb + nwc;          // halfspaces, boxes, capsules; spheres follow
-    const int full = (nws / kWave) * kWave;  // spheres in whole chunks of 64 (sphere steps)
-    const int npo = no + (nws - full);       // pair-step obstacles: no, then the last nws % 64 spheres
-    const int total = npo * ng;              // world primitives are staged in LDS: far below 2^31 / 64
-    // Flops of the first p1 pair steps' pairs (counted once, where the scan stops).
-    auto coarse_flops = [&](int p1) {
-        return pairs_in(0, p1, ng, 0, nwh) * kFlopsSH + pairs_in(0, p1, ng, nwh, nwh + nwb) * kFlopsSB +
-               pairs_in(0, p1, ng, nwh + nwb, no) * kFlopsSC + pairs_in(0, p1, ng, no, npo) * kFlopsSS;
-    };
+    const int total = no * ng;               // world primitives are staged in LDS: far below 2^31 / 64
     int qn = 0;
     // Refine as soon as one pair is queued, then wait for twice as many after each clear
     // refinement: a colliding configuration exits early, a free one refines in full batches.
@@ -614,10 +606,10 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
         qn += static_cast<int>(__popcll(mask));
     };
-    // Pair steps (halfspaces, boxes, capsules, then the spheres past the last whole chunk): pair
-    // p = o * ng + g (obstacle-major), lane l takes p0 + l, and (o, g) advance by kWave pairs per
-    // step without a division (ng <= 64).  Sphere chunks (the many): lanes over a chunk of 64
-    // world spheres, each lane testing its sphere against every group sphere (coarse_clear's
+    // Pair steps (halfspaces, boxes, capsules): pair p = o * ng + g (obstacle-major), lane l takes
+    // p0 + l, and (o, g) advance by kWave pairs per step without a division (ng <= 64).  Sphere
+    // chunks (the many): lanes over a chunk of 64 world spheres (the last one partial), each lane
+    // testing its sphere against every group sphere (coarse_clear's
     // sphere test; independent LDS broadcasts, no per-group ballot) into a mask of groups, whose
     // set bits are then queued one per lane per step.  Steps fill the queue up to flush_at; one
     // refinement site serves both.
@@ -629,18 +621,17 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         o = lane / ng;
         g = lane - (lane / ng) * ng;
     }
-    int pstep = 0, k0 = 0, chunks = 0;  // chunks: sphere chunks whose masks are computed
+    int pstep = 0, k0 = 0, stests = 0;  // stests: sphere-group tests done
     unsigned long long gm = 0;         // this lane's groups still to queue (chunk k0)
     bool loaded = false;
     for (;;) {
         for (; pstep < psteps && qn < flush_at; ++pstep) {
             bool dirty = false;
             int entry = 0;
-            if (o < npo) {
-                const int og = (o < no) ? o : o + full;
+            if (o < no) {
                 const float4 c = CW[g];
-                dirty = !coarse_clear(M, og, c, c.w);
-                entry = (og << 6) | g;
+                dirty = !coarse_clear(M, o, c, c.w);
+                entry = (o << 6) | g;
             }
             o += dq;
             g += dr;
@@ -648,18 +639,21 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
             push(dirty, entry);
         }
         if (pstep >= psteps) {
-            while (k0 < full && qn < flush_at) {
+            while (k0 < nws && qn < flush_at) {
                 if (!loaded) {
-                    const float4 w = M.wsph[k0 + lane];
-                    const float rw = fabsf(w.w);
                     gm = 0;
-                    for (int sg = 0; sg < ng; ++sg) {
-                        const float4 c = CW[sg];
-                        const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
-                        if (!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw)) gm |= 1ull << sg;
+                    if (k0 + lane < nws) {
+                        const float4 w = M.wsph[k0 + lane];
+                        const float rw = fabsf(w.w);
+                        for (int sg = 0; sg < ng; ++sg) {
+                            const float4 c = CW[sg];
+                            const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+                            if (!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw))
+                                gm |= 1ull << sg;
+                        }
                     }
                     loaded = true;
-                    ++chunks;
+                    stests += min(kWave, nws - k0) * ng;
                 }
                 if (!wave_any(gm != 0)) {
                     k0 += kWave;
@@ -671,16 +665,15 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
                 gm &= gm - 1ull;
             }
         }
-        const bool all = pstep >= psteps && k0 >= full;
+        const bool all = pstep >= psteps && k0 >= nws;
         if (qn > 0 && refine()) {
-            flops += coarse_flops(min(pstep * kWave, total)) +
-                     static_cast<unsigned long long>(chunks) * kWave * ng * kFlopsSS;
+            flops += coarse_flops_of(M, min(pstep * kWave, total), ng) + static_cast<unsigned long long>(stests) * kFlopsSS;
             return done(true);
         }
         if (all) break;
         flush_at = min(2 * flush_at, kWave);
     }
-    flops += coarse_flops(total) + static_cast<unsigned long long>(full) * ng * kFlopsSS;
+    flops += coarse_flops_of(M, total, ng) + static_cast<unsigned long long>(nws) * ng * kFlopsSS;
     for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
         const int k = p0 + lane;
         bool h = false;
@@ -747,9 +740,7 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
     // ---- unit coarse stage: wave_culled's steps against the unit spheres, into LU
     const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
     const int no = nwh + nwb + nwc;
-    const int full = (nws / kWave) * kWave;
-    const int npo = no + (nws - full);
-    const int total = npo * ng;
+    const int total = no * ng;
     int nl = 0;
     auto push = [&](bool dirty, int entry) -> bool {
         const unsigned long long mask = __ballot(dirty);
@@ -765,35 +756,36 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         for (int p0 = 0; p0 < total; p0 += kWave) {
             bool dirty = false;
             int entry = 0;
-            if (o < npo) {
-                const int og = (o < no) ? o : o + full;
+            if (o < no) {
                 const float4 c = CU[g];
-                dirty = !coarse_clear(M, og, c, c.w);
-                entry = (og << 6) | g;
+                dirty = !coarse_clear(M, o, c, c.w);
+                entry = (o << 6) | g;
             }
             o += dq;
             g += dr;
             if (g >= ng) { g -= ng; ++o; }
             if (!push(dirty, entry)) {
-                flops += coarse_flops_of(M, min(p0 + kWave, total), ng, npo);
+                flops += coarse_flops_of(M, min(p0 + kWave, total), ng);
                 return done(-1);
             }
         }
-        flops += coarse_flops_of(M, total, ng, npo);
+        flops += coarse_flops_of(M, total, ng);
     }
-    // Sphere chunks: each lane tests its world sphere against every unit sphere (independent LDS
-    // broadcasts and distance tests, no per-group ballot) into a mask of groups, then the few set
-    // bits are pushed, one per lane per pass.
-    for (int k0 = 0; k0 < full; k0 += kWave) {
-        const float4 w = M.wsph[k0 + lane];
-        const float rw = fabsf(w.w);
+    // Sphere chunks (the last one partial): each lane tests its world sphere against every unit
+    // sphere (independent LDS broadcasts and distance tests, no per-group ballot) into a mask of
+    // groups, then the few set bits are pushed, one per lane per pass.
+    for (int k0 = 0; k0 < nws; k0 += kWave) {
         unsigned long long gm = 0;
-        for (int sg = 0; sg < ng; ++sg) {
-            const float4 c = CU[sg];
-            const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
-            if (!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw)) gm |= 1ull << sg;
+        if (k0 + lane < nws) {
+            const float4 w = M.wsph[k0 + lane];
+            const float rw = fabsf(w.w);
+            for (int sg = 0; sg < ng; ++sg) {
+                const float4 c = CU[sg];
+                const float dx = c.x - w.x, dy = c.y - w.y, dz = c.z - w.z;
+                if (!beyond(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), c.w + rw)) gm |= 1ull << sg;
+            }
         }
-        flops += static_cast<unsigned long long>(kWave) * ng * kFlopsSS;
+        flops += static_cast<unsigned long long>(min(kWave, nws - k0)) * ng * kFlopsSS;
         while (wave_any(gm != 0)) {
             const bool has = gm != 0;
             const int sg = has ? static_cast<int>(__builtin_ctzll(gm)) : 0;
