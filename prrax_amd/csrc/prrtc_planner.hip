@@ -720,7 +720,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             __syncthreads();
             }
             int nl, ku, cps, total = 0;
-            bool deal;
+            bool deal, once;
             if (kind == 0) {
                 nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
                 ku = unit_size(nl, G, W * nwaves, fb);
@@ -730,6 +730,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 // (claim counter per explorer), which balances the load and skips edges known to
                 // collide.
                 deal = static_cast<long long>(nl) * G < W * nwaves;
+                once = static_cast<long long>(nl) * cps <= W * nwaves;
             } else {
                 nl = compact(S, [&](int e) { return newi[e] >= 0; }, list, scan);
                 int nsub = 0;
@@ -746,6 +747,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 // lowers the explorer's cap to j, so units past the first failure stop being
                 // handed out.
                 deal = static_cast<long long>(nsub) * G < W * nwaves;
+                once = static_cast<long long>(nsub) * cps <= W * nwaves;
                 if (deal) {
                     for (int k0 = 0; k0 < nl; k0 += nt) {
                         const int k = k0 + tid;
@@ -876,6 +878,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     }
                 }
                 wave_sync();
+                // Units no more than waves: a wave that has checked one unit stops, so that the
+                // units left go to the waves that have not started (members still in an NN scan)
+                // instead of to a second turn of an early wave.
+                if (once && !deal) break;
             }
         }
         if (ended) break;
