@@ -53,7 +53,7 @@ def tuning_sample(contexts: Sequence[dict], n: int = 96, n_heavy: int = 16) -> l
 
 
 def autotune(starts, goals, collision_context, *, max_nodes_per_query=None, samples: Sequence[int] = (128,),
-             threads: Sequence[int] = (128, 256), pass1_rounds: Sequence[int] = (24,),
+             threads: Sequence[int] = (256, 512), pass1_rounds: Sequence[int] = (24,),
              team_max: Sequence[int] = (0,), n_sample: int = 96, repeats: int = 1,
              log=None, **plan_kw) -> TuneResult:
     """Times every candidate on a tuning sample of the batch and returns the fastest whose solve

@@ -1282,7 +1282,10 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail_with(PRRTC_E_HIP, "hipGetDevice failed");
     const DevInfo& di = dev_info(dev);
-    int nt = (launch && launch->threads > 0) ? launch->threads : 256;
+    // 512 threads (8 waves) per work-group by default: the planner kernels take 256 VGPRs, so one
+    // such work-group fills a CU, and a team of them has half the members (half the per-member
+    // redundant work and barrier arrivals) of 256-thread work-groups.
+    int nt = (launch && launch->threads > 0) ? launch->threads : 512;
     if (nt % kWave || nt > 512) return fail_with(PRRTC_E_INVALID, "planner threads must be a multiple of 64, <= 512");
 
     PlanArgs a;
