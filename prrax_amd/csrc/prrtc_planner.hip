@@ -278,11 +278,14 @@ constexpr unsigned long long kNoKey = ~0ull;
 // Waypoints per work unit of a check phase over `edges` edges of G waypoints (a unit is ku
 // consecutive waypoints of one edge, checked in order by one wave after one FK of all of them;
 // a collision skips the rest of the unit): the smallest power of two ku <= fb whose units are no
-// more than the team's waves, so that no wave takes a second unit (a second claim and FK); fb
-// when even fb-waypoint units outnumber the waves.  Uniform over the team.
+// more than the cap below; fb when even fb-waypoint units exceed it.  Uniform over the team.
 __device__ __forceinline__ int unit_size(int edges, int G, int waves, int fb) {
+    // Large teams (the latency-bound tail of a batch): units fit the waves, one each.  Small
+    // teams and single work-groups (many queries at once, throughput-bound): up to two units per
+    // wave, which balances better than one larger unit.
+    const long long cap = (waves <= 256) ? 2ll * waves : static_cast<long long>(waves);
     int ku = 1;
-    while (ku < fb && static_cast<long long>(edges) * ((G + ku - 1) / ku) > waves) ku <<= 1;
+    while (ku < fb && static_cast<long long>(edges) * ((G + ku - 1) / ku) > cap) ku <<= 1;
     return min(ku, fb);
 }
 
