@@ -688,17 +688,18 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
 }
 
 // Verdict of a sub-unit of nu <= kUnitMax configurations whose FK frames are at T + c (c < nu):
-// k + 1 if configuration k is the first that collides, 0 if all are free, -1 if the candidate
-// list overflowed (the caller then checks them one by one).  Needs a cull area with kUnitCap
-// candidate slots.  Consecutive waypoints of one edge are close, so one coarse pass serves all
-// of them: group g's unit sphere has centre C_g (midpoint of its nu transformed centres) and
+// k + 1 if configuration k collides (the first of those found colliding: the scan stops at the
+// first wave step with a world hit, so an earlier configuration may collide too), 0 if all are
+// free, -1 if the candidate list overflowed (the caller then checks them one by one).  Needs a
+// cull area with kUnitCap candidate slots.  Consecutive waypoints of one edge are close, so one
+// coarse pass serves all of them: group g's unit sphere has centre C_g (midpoint of its nu transformed centres) and
 // radius R_g + cut + pad + the half-diagonal of the box around them (wave_culled's coarse limit
 // plus a bound on max_c |p_c - C_g|).  The coarse tests are 1-Lipschitz in the centre, so a pair
 // the unit sphere clears is cleared by every configuration's own group sphere (triangle
 // inequality); the pairs it cannot clear form a candidate list.  The candidates of all
 // configurations are then tested together (wave_culled's coarse and fine tests, 64 pairs per wave
-// step), and the self pairs of the configurations before the first world hit one by one.
-// Verdicts are wave_culled's.
+// step, stopping at the first step with a hit), and, if none hits the world, the self pairs of
+// each configuration in order.  Whether any configuration collides is wave_culled's verdict.
 template <bool MARGIN>
 __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -838,14 +839,19 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
             }
         }
         wave_sync();
+        if (wave_any(hm != 0u)) break;  // a configuration collides: the sub-unit's verdict is known
     }
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) hm |= static_cast<unsigned>(__shfl_xor(static_cast<int>(hm), o, kWave));
-    const int first = hm ? __builtin_ctz(hm) : nu;  // the first configuration with a world hit
-    flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints * (first < nu ? first + 1 : nu);
-    // ---- self pairs of the configurations before it, in order
+    if (hm) {
+        const int first = __builtin_ctz(hm);  // the first of the colliding configurations found
+        flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints * (first + 1);
+        return done(first + 1);
+    }
+    flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints * nu;
+    // ---- no world hit: the self pairs of each configuration, in order
     if (M.n_pairs > 0) {
-        for (int u = 0; u < first; ++u) {
+        for (int u = 0; u < nu; ++u) {
             const float* Tu = T + u;
             for (int s0 = 0; s0 < nrs; s0 += kWave) {
                 const int s = s0 + lane;
@@ -880,7 +886,7 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
             wave_sync();
         }
     }
-    return done(first < nu ? first + 1 : 0);
+    return done(0);
 }
 
 // The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform; the
