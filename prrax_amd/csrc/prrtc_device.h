@@ -235,8 +235,13 @@ __device__ __forceinline__ void fk_lane(const Model& M, const LaneScratch& L) {
     const int nt = L.nt;
     Quat pq = {1.0f, 0.0f, 0.0f, 0.0f};
     float ptx = 0.0f, pty = 0.0f, ptz = 0.0f;
+    // The next joint's record is loaded before this joint's frame is stored, so its reads are
+    // not ordered behind the stores and overlap this joint's arithmetic.
+    JointRec Jn{};
+    if (M.n_joints > 0) Jn = M.joints[0];
     for (int j = 0; j < M.n_joints; ++j) {
-        const JointRec& J = M.joints[j];
+        const JointRec J = Jn;
+        if (j + 1 < M.n_joints) Jn = M.joints[j + 1];
         float theta;
         if (J.mimic >= 0) theta = __builtin_fmaf(L.q[J.mimic * nt], J.mul, J.off);
         else if (J.act >= 0) theta = L.q[J.act * nt];
