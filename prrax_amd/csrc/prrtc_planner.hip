@@ -214,20 +214,26 @@ __device__ __forceinline__ float sample_dim(const char* tab, const unsigned char
 }
 
 // ---------------------------------------------------------------------------------- helpers
+// Explorers e < S with pred(e), in increasing order, into list; returns their count.  Wave 0
+// does it alone, one ballot per 64 explorers (no block-wide scan), and publishes the count in
+// scan[slot] behind one work-group barrier.  Two calls in a row must use different slots (a
+// wave may still be reading the first count when wave 0 writes the second).
 template <class F>
-__device__ __forceinline__ int compact(int S, F pred, int* list, int* scan) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    int base = 0;
-    for (int e0 = 0; e0 < S; e0 += nt) {
-        const int e = e0 + tid;
-        const int ok = (e < S && pred(e)) ? 1 : 0;
-        int tot;
-        const int pre = block_excl_scan(ok, scan, &tot);
-        if (ok) list[base + pre] = e;
-        base += tot;
+__device__ __forceinline__ int compact(int S, F pred, int* list, int* scan, int slot = 19) {
+    const int tid = threadIdx.x;
+    if (tid < kWave) {
+        int base = 0;
+        for (int e0 = 0; e0 < S; e0 += kWave) {
+            const int e = e0 + tid;
+            const bool ok = e < S && pred(e);
+            const unsigned long long m = __ballot(ok);
+            if (ok) list[base + static_cast<int>(__popcll(m & ((1ull << tid) - 1ull)))] = e;
+            base += static_cast<int>(__popcll(m));
+        }
+        if (tid == 0) scan[slot] = base;
     }
     __syncthreads();
-    return base;
+    return scan[slot];
 }
 
 __device__ __forceinline__ void radius_shrink(float& r, float alpha, float r0, float rmin) {
@@ -476,8 +482,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     // part of the tree; otherwise members take whole tiles in turn.  Partial results merge
     // through a 64-bit minimum of (squared distance, index), which equals the serial scan's.
     auto nn_phase = [&](auto tree, auto qpt, unsigned long long* keys, int sz0, int sz1) {
-        const int n0 = compact(S, [&](int e) { return tree(e) == 0; }, list, scan);
-        const int n1 = compact(S, [&](int e) { return tree(e) == 1; }, list + n0, scan);
+        const int n0 = compact(S, [&](int e) { return tree(e) == 0; }, list, scan, 18);
+        const int n1 = compact(S, [&](int e) { return tree(e) == 1; }, list + n0, scan, 19);
         const int nt0 = (n0 + kEC - 1) / kEC, ntl = nt0 + (n1 + kEC - 1) / kEC;
         if (ntl == 0) return;
         int first = m, stride = W, parts = 1, part = 0;
