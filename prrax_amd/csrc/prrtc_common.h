@@ -149,6 +149,33 @@ __device__ __forceinline__ int block_excl_scan(int v, int* scratch, int* total) 
     return res;
 }
 
+// Exclusive prefix sums of two per-thread values over the work-group (thread order), with their
+// totals, behind two barriers (each wave scans both values; the waves' totals go through LDS).
+// scratch needs 2 * (waves) ints (at most 16).
+__device__ __forceinline__ void block_excl_scan2(int v0, int v1, int* scratch, int& p0, int& p1, int& tot0,
+                                                 int& tot1) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wid = tid / kWave, nw = (blockDim.x + kWave - 1) / kWave;
+    const int i0 = wave_incl_scan(v0), i1 = wave_incl_scan(v1);
+    if (lane == kWave - 1) {
+        scratch[wid] = i0;
+        scratch[8 + wid] = i1;
+    }
+    __syncthreads();
+    int b0 = 0, b1 = 0, t0 = 0, t1 = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int c0 = scratch[w], c1 = scratch[8 + w];
+        if (w < wid) { b0 += c0; b1 += c1; }
+        t0 += c0;
+        t1 += c1;
+    }
+    p0 = b0 + i0 - v0;
+    p1 = b1 + i1 - v1;
+    tot0 = t0;
+    tot1 = t1;
+    __syncthreads();  // the scratch words may be rewritten by the next scan
+}
+
 // ---------------------------------------------------------------------------------- staging
 // Stages robot + world of query q into LDS and fills M.  Must be called by the whole
 // work-group; ends with a barrier.  pair_cap is the LDS capacity for self pairs.

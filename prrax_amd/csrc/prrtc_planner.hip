@@ -650,9 +650,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     const int e = e0 + tid;
                     const bool fr = e < S && state[e] == 1;
                     const int t = fr ? side[e] : 0;
-                    int tot0, tot1;
-                    const int p0 = block_excl_scan((fr && t == 0) ? 1 : 0, scan, &tot0);
-                    const int p1 = block_excl_scan((fr && t == 1) ? 1 : 0, scan, &tot1);
+                    int p0, p1, tot0, tot1;
+                    block_excl_scan2((fr && t == 0) ? 1 : 0, (fr && t == 1) ? 1 : 0, scan, p0, p1, tot0, tot1);
                     if (e < S) {
                         int ni = -1;
                         if (fr) {
@@ -910,9 +909,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 const bool act = e < S && newi[e] >= 0;
                 const int t = act ? side[e] : 0;
                 const int n_ok = act ? nok[e] : 0;
-                int tot0, tot1;
-                const int p0 = block_excl_scan(t == 0 ? n_ok : 0, scan, &tot0);
-                const int p1 = block_excl_scan(t == 1 ? n_ok : 0, scan, &tot1);
+                int p0, p1, tot0, tot1;
+                block_excl_scan2(t == 0 ? n_ok : 0, t == 1 ? n_ok : 0, scan, p0, p1, tot0, tot1);
                 if (act) {
                     const int b = t ? base1 + p1 : base0 + p0;
                     const bool over = b + n_ok > N;
