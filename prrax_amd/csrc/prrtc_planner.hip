@@ -741,13 +741,15 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 once = static_cast<long long>(nl) * cps <= W * nwaves;
             } else {
                 nl = compact(S, [&](int e) { return newi[e] >= 0; }, list, scan);
-                int nsub = 0;
-                for (int k0 = 0; k0 < nl; k0 += nt) {
-                    const int k = k0 + tid;
-                    int tot;
-                    block_excl_scan(k < nl ? next[list[k]] : 0, scan, &tot);
-                    nsub += tot;
+                // sub-edges of all walks: summed by wave 0, published behind one barrier
+                if (tid < kWave) {
+                    unsigned acc = 0;
+                    for (int k = tid; k < nl; k += kWave) acc += static_cast<unsigned>(next[list[k]]);
+                    acc = wave_sum_u32(acc);
+                    if (tid == 0) scan[17] = static_cast<int>(acc);
                 }
+                __syncthreads();
+                const int nsub = scan[17];
                 ku = unit_size(nsub, G, W * nwaves, fb);
                 cps = (G + ku - 1) / ku;
                 // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers
