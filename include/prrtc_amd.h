@@ -122,7 +122,7 @@ typedef struct {
                              work-groups, else 0: every query starts on a team) */
     int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (512) */
     int32_t epoch_rounds; /* planner: most rounds a query runs between two re-formings of the teams of
-                             the team pass (epochs run 4, 8, 16, ... rounds up to this); 0 = auto (32) */
+                             the team pass (epochs run 4, 8, 16, ... rounds up to this); 0 = auto (64) */
 } prrtc_launch;
 
 /* Workspace bytes a planner call needs (radii + per-query explorer state + queue counters). */

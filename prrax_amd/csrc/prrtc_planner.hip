@@ -1355,7 +1355,7 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     // a smaller batch starts directly on teams of >= 2 work-groups.
     a.pass1_rounds = (launch && launch->pass1_rounds >= 0) ? launch->pass1_rounds : (2 * B <= a.slots ? 0 : 24);
     a.wmax = (launch && launch->team_max > 0) ? launch->team_max : 512;
-    a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 32;
+    a.epoch_rounds = (launch && launch->epoch_rounds > 0) ? launch->epoch_rounds : 64;
     const unsigned long long shape = static_cast<unsigned long long>(a.slots) |
                                      (static_cast<unsigned long long>(nt) << 16) |
                                      (static_cast<unsigned long long>(a.pass1_rounds & 0xffff) << 32) |
