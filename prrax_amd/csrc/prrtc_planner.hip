@@ -733,12 +733,12 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 nl = compact(S, [&](int e) { return state[e] == 1; }, list, scan);
                 ku = unit_size(nl, G, W * nwaves, fb);
                 cps = (G + ku - 1) / ku;
-                // Fewer waypoints than waves: a fixed round-robin deal of one-waypoint units (no
-                // claiming; nothing is left to skip).  Otherwise units are claimed dynamically
-                // (claim counter per explorer), which balances the load and skips edges known to
-                // collide.
-                deal = static_cast<long long>(nl) * G < W * nwaves;
+                // Units no more than waves: a fixed deal, unit us to wave us (no claiming; a unit
+                // of an edge already known to collide is skipped).  Otherwise units are claimed
+                // dynamically (claim counter per explorer), which balances the load and skips
+                // edges known to collide.
                 once = static_cast<long long>(nl) * cps <= W * nwaves;
+                deal = once || static_cast<long long>(nl) * G < W * nwaves;
             } else {
                 nl = compact(S, [&](int e) { return newi[e] >= 0; }, list, scan);
                 // sub-edges of all walks: summed by wave 0, published behind one barrier
@@ -752,12 +752,12 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 const int nsub = scan[17];
                 ku = unit_size(nsub, G, W * nwaves, fb);
                 cps = (G + ku - 1) / ku;
-                // Fewer waypoints than waves: a fixed round-robin deal (unit offsets of the walkers
-                // in aux0).  Otherwise units are claimed dynamically, and a collision at sub-edge j
-                // lowers the explorer's cap to j, so units past the first failure stop being
-                // handed out.
-                deal = static_cast<long long>(nsub) * G < W * nwaves;
+                // Units no more than waves: a fixed deal, unit us to wave us (unit offsets of the
+                // walkers in aux0; units past a walk's first failure found so far are skipped).
+                // Otherwise units are claimed dynamically, and a collision at sub-edge j lowers the
+                // explorer's cap to j, so units past the first failure stop being handed out.
                 once = static_cast<long long>(nsub) * cps <= W * nwaves;
+                deal = once || static_cast<long long>(nsub) * G < W * nwaves;
                 if (deal) {
                     for (int k0 = 0; k0 < nl; k0 += nt) {
                         const int k = k0 + tid;
@@ -888,10 +888,9 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     }
                 }
                 wave_sync();
-                // Units no more than waves: a wave that has checked one unit stops, so that the
-                // units left go to the waves that have not started (members still in an NN scan)
-                // instead of to a second turn of an early wave.
-                if (once && !deal) break;
+                // Units no more than waves: they are dealt (unit us to wave us), so a wave checks
+                // at most one and no claim counters are read or raised.
+                if (once) break;
             }
         }
         if (ended) break;
