@@ -539,6 +539,23 @@ __device__ __forceinline__ unsigned long long coarse_flops_of(const Model& M, in
            pairs_in(0, p1, ng, nwh + nwb, no) * kFlopsSC;
 }
 
+// Self pairs k0 + lane and k0 + kWave + lane against the transformed spheres FP (two chunks per
+// wave step: their loads are issued together, one ballot for both).
+template <bool MARGIN>
+__device__ __forceinline__ bool self_pairs_2(const Model& M, const float4* FP, int k0, int lane) {
+    const int k = k0 + lane, k2 = k0 + kWave + lane;
+    bool h = false;
+    if (k < M.n_pairs) {
+        const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
+        h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+    }
+    if (k2 < M.n_pairs) {
+        const float4 a = FP[M.pairs[k2] & 0xffffu], b = FP[M.pairs[k2] >> 16];
+        h |= MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+    }
+    return h;
+}
+
 template <bool MARGIN>
 __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -679,14 +696,9 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         flush_at = min(2 * flush_at, kWave);
     }
     flops += coarse_flops_of(M, total, ng) + static_cast<unsigned long long>(nws) * ng * kFlopsSS;
-    for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
-        const int k = p0 + lane;
-        bool h = false;
-        if (k < M.n_pairs) {
-            const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
-            h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
-        }
-        flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
+    for (int p0 = 0; p0 < M.n_pairs; p0 += 2 * kWave) {
+        const bool h = self_pairs_2<MARGIN>(M, FP, p0, lane);
+        flops += static_cast<unsigned long long>(min(2 * kWave, M.n_pairs - p0)) * kFlopsPair;
         if (wave_any(h)) return done(true);
     }
     return done(false);
@@ -878,14 +890,9 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
                 flops += static_cast<unsigned long long>(__popcll(__ballot(act))) * kFlopsXf;
             }
             wave_sync();
-            for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
-                const int k = p0 + lane;
-                bool h = false;
-                if (k < M.n_pairs) {
-                    const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
-                    h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
-                }
-                flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
+            for (int p0 = 0; p0 < M.n_pairs; p0 += 2 * kWave) {
+                const bool h = self_pairs_2<MARGIN>(M, FP, p0, lane);
+                flops += static_cast<unsigned long long>(min(2 * kWave, M.n_pairs - p0)) * kFlopsPair;
                 if (wave_any(h)) return done(u + 1);
             }
             wave_sync();
