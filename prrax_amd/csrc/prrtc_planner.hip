@@ -467,8 +467,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
     // kernel boundary) and never change: NN scans read them through the caches.
     const int coh_a = sc[SC_SIZE_A], coh_b = sc[SC_SIZE_B];
     unsigned long long evals = 0, flops = 0;  // lane 0 of each wave counts
-    // Per-phase wall time (member 0, thread 0): up to B1, B3 (extend checks and connect NN), the
-    // commit, (slot 3 unused), B7, B8.
+    // Per-phase wall time (member 0, thread 0): up to B1, B3 (extend checks), the commit, B4
+    // (connect NN), B7 (connect prep and walks), B8.
     unsigned long long ph[6] = {0, 0, 0, 0, 0, 0};
     unsigned long long tph = wall_clock64();
     auto mark = [&](int k) {
@@ -519,10 +519,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             __syncthreads();
         }
     };
-    // Connect searches of this round's candidates (all = every candidate, else the successful
-    // extends) against the other tree's first sz0 / sz1 nodes.
-    auto connect_nn = [&](bool all, int sz0, int sz1) {
-        nn_phase([&](int e) { return (all ? state[e] == 1 : newi[e] >= 0) ? 1 - side[e] : -1; },
+    // Connect searches of this round's successful extends against the other tree's first sz0 /
+    // sz1 nodes.
+    auto connect_nn = [&](int sz0, int sz1) {
+        nn_phase([&](int e) { return newi[e] >= 0 ? 1 - side[e] : -1; },
                  [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, sz0, sz1);
     };
     const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
@@ -616,12 +616,6 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         __syncthreads();
-        // ---- phase 4 (early, teams): connect nearest neighbour of every candidate in the other
-        // tree as it stands at the start of the round -- the interleaving in which each
-        // explorer's connect search precedes the other explorers' appends of this round.  A team
-        // computes it for all candidates here, so no team barrier is needed after the commit; a
-        // single work-group searches after the commit, for the successful extends only.
-        if (W > 1) connect_nn(true, size_a, size_b);
         // ---- phases 2 (kind 0) and 5 (kind 1) share one check loop, so that the FK and the
         // collision checks are instantiated once (the kernels' register budget); the phase-3
         // commit and the connect prep run between them, after B3.
@@ -698,7 +692,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     if (sc[SC_OVERFLOW]) sc[SC_STATUS] = PRRTC_STATUS_TREE_FULL;
                 }
             }
-            // No team barrier: the connect walk reads only nodes that existed at the start of the round
+            // The connect NN and walk read only nodes that existed at the start of the round
             // (appends and radius updates are seen from the next round, after B7 / B8).
             __syncthreads();
             mark(2);
@@ -706,7 +700,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 ended = true;
                 break;
             }
-            if (W == 1) connect_nn(false, size_a, size_b);
+            // ---- phase 4: connect nearest neighbour of each successful extend in the other tree
+            // as it stood at the start of the round (its first size_a / size_b nodes) -- the
+            // interleaving in which each explorer's connect search precedes the other explorers'
+            // appends of this round.  Only the extends that passed are searched (most collide);
+            // a team merges its members' partial results before the connect prep reads them.
+            connect_nn(size_a, size_b);
+            team_barrier(team);  // B4: connect NN keys merged
+            mark(3);
             for (int e = tid; e < S; e += nt) {
                 if (newi[e] < 0) continue;
                 const float* O = side[e] ? TA : TB;
