@@ -8,8 +8,8 @@
 //   2  FK + collision of every extend edge (G waypoints each)  prrtc_planner.cu:883-909
 //   3  commit: all radius shrinks, then appends in explorer order with radius grows
 //                                                         prrtc_planner.cu:911-947, 46-83
-//   4  connect nearest neighbour in the other tree as it stood at the round's start (run
-//      alongside phase 2), step count and step vector      prrtc_planner.cu:949-994
+//   4  connect nearest neighbour of each successful extend in the other tree as it stood at the
+//      round's start, step count and step vector          prrtc_planner.cu:949-994
 //   5  connect sub-edges, checked in parallel work units; each explorer's first failing sub-edge
 //      is an order-independent minimum
 //                                                         prrtc_planner.cu:996-1018
