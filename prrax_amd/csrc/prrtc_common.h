@@ -443,20 +443,51 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
             }
         }
     }
-    // Wave minimum in lex_less order: the smallest distance (distances are non-negative, so
-    // their bit patterns order as unsigned integers; the device library's wave reduction), then
-    // the smallest index among the lanes holding it (one lane, unless distances tie exactly).
+    // Wave minimum of every query in lex_less order (smallest distance, then lowest index), by
+    // halving: at each step a lane keeps half of its queries and merges the partner lane's values
+    // of the same queries (lane ^ 32, ^ 16, ^ 8, ^ 4); lanes 4q .. 4q + 3 then hold query q and
+    // finish with two more exchanges.  The minimum is associative, so the order does not matter.
+    {
+        auto merge = [&](float kd, int ki, float od, int oi, float& d, int& i) {
+            const bool take = lex_less(od, oi, kd, ki);
+            d = take ? od : kd;
+            i = take ? oi : ki;
+        };
+        float d8[8], d4[4], d2[2], d1;
+        int i8[8], i4[4], i2[2], i1;
+        const bool u32 = (lane & 32) != 0, u16 = (lane & 16) != 0, u8 = (lane & 8) != 0, u4 = (lane & 4) != 0;
 #pragma unroll
-    for (int c = 0; c < kEC; ++c) {
-        const unsigned bits = __float_as_uint(bd[c]);
-        const unsigned mbits = __ockl_wfred_min_u32(bits);
-        const unsigned long long tie = __ballot(bits == mbits);
-        int idx;
-        if (__popcll(tie) == 1) idx = __builtin_amdgcn_readlane(bi[c], static_cast<int>(__ffsll(static_cast<long long>(tie)) - 1));
-        else idx = static_cast<int>(__ockl_wfred_min_u32(bits == mbits ? static_cast<unsigned>(bi[c]) : 0xffffffffu));
-        if (lane == 0) {
-            red_d[c * 16 + wid] = __uint_as_float(mbits);
-            red_i[c * 16 + wid] = idx;
+        for (int j = 0; j < 8; ++j) {
+            const float od = __shfl_xor(u32 ? bd[j] : bd[8 + j], 32, kWave);
+            const int oi = __shfl_xor(u32 ? bi[j] : bi[8 + j], 32, kWave);
+            merge(u32 ? bd[8 + j] : bd[j], u32 ? bi[8 + j] : bi[j], od, oi, d8[j], i8[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float od = __shfl_xor(u16 ? d8[j] : d8[4 + j], 16, kWave);
+            const int oi = __shfl_xor(u16 ? i8[j] : i8[4 + j], 16, kWave);
+            merge(u16 ? d8[4 + j] : d8[j], u16 ? i8[4 + j] : i8[j], od, oi, d4[j], i4[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float od = __shfl_xor(u8 ? d4[j] : d4[2 + j], 8, kWave);
+            const int oi = __shfl_xor(u8 ? i4[j] : i4[2 + j], 8, kWave);
+            merge(u8 ? d4[2 + j] : d4[j], u8 ? i4[2 + j] : i4[j], od, oi, d2[j], i2[j]);
+        }
+        {
+            const float od = __shfl_xor(u4 ? d2[0] : d2[1], 4, kWave);
+            const int oi = __shfl_xor(u4 ? i2[0] : i2[1], 4, kWave);
+            merge(u4 ? d2[1] : d2[0], u4 ? i2[1] : i2[0], od, oi, d1, i1);
+        }
+#pragma unroll
+        for (int o = 2; o > 0; o >>= 1) {
+            const float od = __shfl_xor(d1, o, kWave);
+            const int oi = __shfl_xor(i1, o, kWave);
+            merge(d1, i1, od, oi, d1, i1);
+        }
+        if ((lane & 3) == 0) {
+            red_d[(lane >> 2) * 16 + wid] = d1;
+            red_i[(lane >> 2) * 16 + wid] = i1;
         }
     }
     __syncthreads();
