@@ -319,6 +319,7 @@ struct Team {
     int G;           // 1, or the groups members arrive in (member m in group m % G)
     int* error;
     unsigned gen;
+    bool released;   // thread 0: its arrival was the last one (team_arrive), no wait needed
 };
 
 // Waits until every global access this thread issued has completed; also a compiler barrier.
@@ -346,7 +347,11 @@ __device__ __forceinline__ bool count_in(unsigned* cnt, unsigned n, bool reset) 
     }
     return true;
 }
-__device__ __forceinline__ void team_barrier(Team& t, bool reset = false) {
+// A barrier in two halves: team_arrive counts the work-group in (and opens the next generation
+// if it is the last arrival); team_wait waits for that generation.  Work between the two must
+// not touch memory the other members read or write before the barrier (for instance the next
+// round's samples, which live in the work-group's LDS).
+__device__ __forceinline__ void team_arrive(Team& t, bool reset = false) {
     if (t.W > 1) {
         drain_memory();
         __syncthreads();
@@ -355,9 +360,18 @@ __device__ __forceinline__ void team_barrier(Team& t, bool reset = false) {
             bool last = true;
             if (t.G > 1) last = count_in(t.grp + (t.m % t.G) * kSyncLine, group_size(t.W, t.G, t.m % t.G), reset);
             if (last) last = count_in(t.sync, static_cast<unsigned>(t.G > 1 ? t.G : t.W), reset);
-            if (last) {
-                __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
+            if (last) __hip_atomic_fetch_add((PRRTC_GLOBAL unsigned*)gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            t.released = last;
+        }
+    } else {
+        __syncthreads();
+    }
+}
+__device__ __forceinline__ void team_wait(Team& t) {
+    if (t.W > 1) {
+        if (threadIdx.x == 0) {
+            unsigned* gen = t.sync + kSyncLine;
+            if (!t.released) {
                 unsigned spins = 0;
                 while (__hip_atomic_load((PRRTC_GLOBAL unsigned*)gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
                        t.gen) {
@@ -377,6 +391,11 @@ __device__ __forceinline__ void team_barrier(Team& t, bool reset = false) {
     } else {
         __syncthreads();
     }
+}
+
+__device__ __forceinline__ void team_barrier(Team& t, bool reset = false) {
+    team_arrive(t, reset);
+    team_wait(t);
 }
 
 // Barrier over the whole grid between epochs: as team_barrier, plus agent-scope release/acquire
@@ -526,6 +545,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                  [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, sz0, sz1);
     };
     const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
+    bool sampled = false;  // cand holds this round's samples (drawn during the previous B8)
     int r = r_begin;
     for (;; ++r) {
         if (r > a.max_iterations) {
@@ -573,11 +593,15 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 gstore(&V.g_cfail[e], 0x7fffffff);
             }
         }
-        // This round's samples of every explorer, into cand (free until phase 1c steers them).
-        halton_round(lds + a.off_sampler, r);
-        __syncthreads();
-        for (int e = tid; e < S; e += nt)
-            for (int d = 0; d < dim; ++d) cand[e * dim + d] = sample_dim(lds + a.off_sampler, primes + 16 * e, d);
+        // This round's samples of every explorer, into cand (free until phase 1c steers them),
+        // unless the previous round drew them while waiting at B8.
+        if (!sampled) {
+            halton_round(lds + a.off_sampler, r);
+            __syncthreads();
+            for (int e = tid; e < S; e += nt)
+                for (int d = 0; d < dim; ++d) cand[e * dim + d] = sample_dim(lds + a.off_sampler, primes + 16 * e, d);
+        }
+        sampled = false;
         __syncthreads();
         // ---- phase 1b: nearest neighbour of each explorer's sample in its tree
         nn_phase([&](int e) { return side[e]; },
@@ -958,7 +982,15 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 }
             }
         }
-        team_barrier(team);  // B8: connect nodes are in global memory
+        // B8: connect nodes are in global memory.  Between arriving and waiting, the work-group
+        // draws the next round's samples (LDS only; phase 6 no longer reads cand).
+        team_arrive(team);
+        halton_round(lds + a.off_sampler, r + 1);
+        __syncthreads();
+        for (int e = tid; e < S; e += nt)
+            for (int d = 0; d < dim; ++d) cand[e * dim + d] = sample_dim(lds + a.off_sampler, primes + 16 * e, d);
+        sampled = true;
+        team_wait(team);
         mark(5);
         if (W > 1) {
             if (tid == 0) sc[SC_ERR] = gload(team.error);
