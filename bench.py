@@ -27,7 +27,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (datasheet)
 PEAK_HBM_GBS = 8000.0
 # HBM bytes of one batched step of the default 1024-query workload (plan_pass1 + plan_pass2),
 # from rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) and WRITE_SIZE (x 1.0).
-TRAFFIC_BYTES_PER_STEP = 17.21e9
+TRAFFIC_BYTES_PER_STEP = 16.82e9
 
 METRIC = "plans/sec + p50 ms/plan, batched vs per-query at equal solve rate; FK+CC cfg/s"
 
