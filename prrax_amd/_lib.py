@@ -1,7 +1,8 @@
 """ctypes binding of ``_prrtc_planner_lib.so`` (declarations in ``include/prrtc_amd.h``).
 
-The library is the only compute path: if it is missing or fails to load, every entry point
-raises ``RuntimeError`` (there is no CPU fallback).  Loading mirrors the reference's
+The library is the only compute path: if it is missing, fails to load or was built from other
+sources than the ones next to it (``prrtc_version()`` carries the build's source hash), every
+entry point raises ``RuntimeError`` (there is no CPU fallback).  Loading mirrors the reference's
 ``_load_and_register`` (cuda-rrtc/jax/prrtc.py:113-146): one ``ctypes.CDLL`` of the library
 next to the package, cached for the process.
 """
@@ -107,6 +108,9 @@ def lib() -> ctypes.CDLL:
         )
     L = ctypes.CDLL(str(LIB_PATH))
     i32, sz, P = ctypes.c_int32, ctypes.c_size_t, ctypes.POINTER
+    L.prrtc_version.argtypes = []
+    L.prrtc_version.restype = ctypes.c_char_p
+    _check_fresh(L.prrtc_version().decode())
     L.prrtc_planner_workspace_bytes.argtypes = [i32, i32, i32, i32]
     L.prrtc_planner_workspace_bytes.restype = sz
     L.prrtc_planner_stats_offset.argtypes = [i32, i32, i32, i32]
@@ -136,9 +140,24 @@ def lib() -> ctypes.CDLL:
     L.prrtc_refine_paths.restype = ctypes.c_int
     L.prrtc_last_error.argtypes = []
     L.prrtc_last_error.restype = ctypes.c_char_p
-    L.prrtc_version.argtypes = []
-    L.prrtc_version.restype = ctypes.c_char_p
     return L
+
+
+def library_source_hash(version: str) -> str:
+    """The source hash a ``prrtc_version()`` string carries ("" if none)."""
+    return version.rsplit(" src ", 1)[1].strip() if " src " in version else ""
+
+
+def _check_fresh(version: str) -> None:
+    from . import build
+
+    if not build.source_files():
+        return  # sources not shipped with this install: nothing to compare against
+    want, have = build.source_hash(), library_source_hash(version)
+    if have != want:
+        raise RuntimeError(
+            f"{LIB_PATH} was built from other sources (library {have or 'without a hash'}, sources {want}); "
+            "rebuild it with `python -m prrax_amd.build`")
 
 
 def check(fn: str, rc: int) -> None:

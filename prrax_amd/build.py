@@ -10,6 +10,7 @@ division and square root, denormals kept, and packed FP32 instructions disabled.
 
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -39,9 +40,27 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
+def source_files() -> list[Path]:
+    """Every file the library is compiled from: the HIP sources, their headers and the C ABI."""
+    return sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.h"), *(ROOT / "include").glob("*.h")])
+
+
+def source_hash() -> str:
+    """First 16 hex digits of a SHA-256 over the library's source files (names and contents);
+    the build passes it to ``prrtc_version()``, so a loaded library can be matched to the sources
+    next to it."""
+    h = hashlib.sha256()
+    for f in source_files():
+        h.update(f.relative_to(ROOT).as_posix().encode())
+        h.update(b"\0")
+        h.update(f.read_bytes())
+        h.update(b"\0")
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
-    cmd = ["hipcc", *HIP_FLAGS, *(extra or []), "-I", str(ROOT / "include"), "-I", str(CSRC), "-o", str(LIB) + ".tmp",
-           *map(str, sources())]
+    cmd = ["hipcc", *HIP_FLAGS, f'-DPRRTC_SRC_HASH="{source_hash()}"', *(extra or []), "-I", str(ROOT / "include"),
+           "-I", str(CSRC), "-o", str(LIB) + ".tmp", *map(str, sources())]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

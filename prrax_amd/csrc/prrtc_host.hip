@@ -82,4 +82,9 @@ int check_world(const prrtc_world* w) {
 
 extern "C" const char* prrtc_last_error(void) { return prrtc::g_err.c_str(); }
 
-extern "C" const char* prrtc_version(void) { return "prrtc_amd 0.2.0 (gfx950)"; }
+// PRRTC_SRC_HASH: the build's hash of the library's source files (prrax_amd/build.py).
+#ifndef PRRTC_SRC_HASH
+#define PRRTC_SRC_HASH "unknown"
+#endif
+
+extern "C" const char* prrtc_version(void) { return "prrtc_amd 0.3.0 (gfx950) src " PRRTC_SRC_HASH; }

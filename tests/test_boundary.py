@@ -33,6 +33,18 @@ def test_library_exports_every_declared_symbol():
     assert b"gfx950" in lib.prrtc_version()
 
 
+def test_library_built_from_these_sources():
+    """prrtc_version() carries the hash of the sources the library was compiled from; the
+    loader refuses a library whose hash differs from the sources next to it."""
+    from prrax_amd import build
+
+    have = _lib.library_source_hash(_lib.lib().prrtc_version().decode())
+    assert have == build.source_hash()
+    assert len(have) == 16
+    with pytest.raises(RuntimeError, match="built from other sources"):
+        _lib._check_fresh("prrtc_amd 0.3.0 (gfx950) src 0000000000000000")
+
+
 def test_reference_api_surface():
     for name in ("prrtc_plan", "prrtc_plan_batch", "prrtc_nearest_neighbor", "prrtc_extend", "PRRTCResult"):
         assert hasattr(prrax_amd, name)
