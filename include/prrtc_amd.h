@@ -145,6 +145,22 @@ int prrtc_planner(prrtc_stream_t stream, const float* start, const float* goals,
                   const prrtc_attrs* attrs, const prrtc_outputs* out, const prrtc_launch* launch, void* workspace,
                   size_t workspace_bytes);
 
+/* Bytes of the device array prrtc_planner_noalloc takes its workspace from (per GPU). */
+#define PRRTC_NOALLOC_SCRATCH_BYTES (16u << 20)
+
+/* prrtc_planner without a workspace argument and without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The workspace is a static device array of
+ * PRRTC_NOALLOC_SCRATCH_BYTES on each GPU, held until the process ends; a call whose
+ * prrtc_planner_workspace_bytes(1, dim, max_nodes, num_new_samples) exceeds it returns
+ * PRRTC_E_WORKSPACE (16 MiB covers max_nodes up to 1.5 million at any num_new_samples).
+ * Calls share the array: calls that overlap on one GPU (two streams, or two threads) race.
+ * Results equal prrtc_planner's. */
+int prrtc_planner_noalloc(prrtc_stream_t stream, const float* start, const float* goals, int32_t num_goals,
+                          const float* min_vals, const float* max_vals, const prrtc_robot* robot,
+                          const prrtc_world* world, const prrtc_attrs* attrs, const prrtc_outputs* out,
+                          const prrtc_launch* launch);
+
 /* Batch of independent queries in one persistent launch.  Replaces PrrtcPlannerBatchFfi
  * (prrtc_planner.cu:2007-2061; shared world: world->counts == NULL and world_per_query == 0) and
  * PrrtcPlannerBatchCtxFfi (prrtc_planner.cu:1946-2005; padded per-query worlds, world_per_query=1).

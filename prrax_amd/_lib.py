@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "prrtc_planner_workspace_bytes",
     "prrtc_planner_stats_offset",
     "prrtc_planner",
+    "prrtc_planner_noalloc",
     "prrtc_planner_batch",
     "prrtc_check_configs",
     "prrtc_forward_kinematics",
@@ -118,6 +119,9 @@ def lib() -> ctypes.CDLL:
     L.prrtc_planner.argtypes = [_vp, _vp, _vp, i32, _vp, _vp, P(Robot), P(World), P(Attrs), P(Outputs), P(Launch),
                                 _vp, sz]
     L.prrtc_planner.restype = ctypes.c_int
+    L.prrtc_planner_noalloc.argtypes = [_vp, _vp, _vp, i32, _vp, _vp, P(Robot), P(World), P(Attrs), P(Outputs),
+                                        P(Launch)]
+    L.prrtc_planner_noalloc.restype = ctypes.c_int
     L.prrtc_planner_batch.argtypes = [_vp, i32, _vp, _vp, i32, _vp, _vp, _vp, P(Robot), P(World), i32, _vp, _vp,
                                       _vp, P(Attrs), P(Outputs), P(Launch), _vp, sz]
     L.prrtc_planner_batch.restype = ctypes.c_int

@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "prrtc_amd.h"
@@ -1400,6 +1401,34 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     return PRRTC_OK;
 }
 
+// ---------------------------------------------------------------------------------- no-allocation entry
+// prrtc_planner_noalloc takes its workspace from `scratch`, a file-scope device array: every GPU
+// the library runs on holds its own copy until the process ends.  In host code the array's name
+// is only a stand-in for its device address, so the address is looked up once per device (with
+// that device current) and the kernels receive it as their workspace pointer.  Calls share the
+// array: calls that overlap on one GPU race.  Everything the planner reads from its workspace
+// before writing it (control words, statistics, barrier lines) is cleared by reset_kernel at the
+// start of every call, so a call never sees the previous call's contents.
+static __device__ __align__(256) unsigned char scratch[PRRTC_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+// Device address of this device's `scratch` (looked up on the first call on the device).
+static int scratch_of_device(int dev, void** ptr) {
+    if (dev < 0 || dev >= 64) return fail_with(PRRTC_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail_with(PRRTC_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return PRRTC_OK;
+}
+
 }  // namespace prrtc
 
 using namespace prrtc;
@@ -1422,6 +1451,29 @@ int prrtc_planner(prrtc_stream_t stream, const float* start, const float* goals,
                   size_t workspace_bytes) {
     return planner_impl(reinterpret_cast<hipStream_t>(stream), 1, start, goals, num_goals, nullptr, min_vals, max_vals,
                         robot, world, 0, nullptr, nullptr, nullptr, attrs, out, launch, workspace, workspace_bytes);
+}
+
+int prrtc_planner_noalloc(prrtc_stream_t stream, const float* start, const float* goals, int32_t num_goals,
+                          const float* min_vals, const float* max_vals, const prrtc_robot* robot,
+                          const prrtc_world* world, const prrtc_attrs* attrs, const prrtc_outputs* out,
+                          const prrtc_launch* launch) {
+    if (!attrs) return fail_with(PRRTC_E_INVALID, "attrs/outputs are null");
+    if (attrs->max_nodes < 1 || attrs->max_nodes > (1 << 29)) return fail_with(PRRTC_E_INVALID, "max_nodes out of range");
+    if (attrs->num_new_samples < 1 || attrs->num_new_samples > PRRTC_MAX_SAMPLES)
+        return fail_with(PRRTC_E_INVALID, "num_new_samples must be in [1, 1024]");
+    const size_t need = ws_layout(1, attrs->max_nodes, attrs->num_new_samples).total;
+    if (need > sizeof(scratch))
+        return fail_with(PRRTC_E_WORKSPACE, "max_nodes " + std::to_string(attrs->max_nodes) + " with num_new_samples " +
+                                                std::to_string(attrs->num_new_samples) + " needs " +
+                                                std::to_string(need) + " workspace bytes; prrtc_planner_noalloc holds " +
+                                                std::to_string(sizeof(scratch)) + " (use prrtc_planner)");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail_with(PRRTC_E_HIP, "hipGetDevice failed");
+    void* ws = nullptr;
+    const int rc = scratch_of_device(dev, &ws);
+    if (rc) return rc;
+    return planner_impl(reinterpret_cast<hipStream_t>(stream), 1, start, goals, num_goals, nullptr, min_vals, max_vals,
+                        robot, world, 0, nullptr, nullptr, nullptr, attrs, out, launch, ws, sizeof(scratch));
 }
 
 int prrtc_planner_batch(prrtc_stream_t stream, int32_t batch, const float* starts, const float* goals,
