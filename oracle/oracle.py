@@ -73,6 +73,37 @@ class OcOutputs(ctypes.Structure):
 _LIB = None
 
 
+def bind(L: ctypes.CDLL) -> ctypes.CDLL:
+    """Declares the oracle's C signatures on a loaded library (this one, or a variant of it
+    built with a planted mistake)."""
+    L.oc_fk.argtypes = [ctypes.POINTER(OcContext), _f32p, _f32p]
+    L.oc_in_collision.argtypes = [ctypes.POINTER(OcContext), _f32p]
+    L.oc_in_collision.restype = ctypes.c_int
+    L.oc_in_collision_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)]
+    L.oc_min_distance_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, _f32p]
+    L.oc_sphere_positions.argtypes = [ctypes.POINTER(OcContext), _f32p, _f32p]
+    L.oc_halton.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.oc_halton.restype = ctypes.c_float
+    L.oc_prime_order.argtypes = [ctypes.c_int, ctypes.c_uint, _i32p]
+    L.oc_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
+    L.oc_nearest.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int, _i32p, _f32p]
+    L.oc_dd_shrink.argtypes = [_f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.oc_dd_grow.argtypes = [_f32p, ctypes.c_float]
+    L.oc_dd_skips.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.oc_dd_skips.restype = ctypes.c_int
+    L.oc_balance_side.argtypes = [ctypes.c_int] * 6 + [ctypes.c_float, _i32p]
+    L.oc_balance_side.restype = ctypes.c_int
+    L.oc_steer.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p]
+    L.oc_connect_steps.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p]
+    L.oc_connect_steps.restype = ctypes.c_int
+    L.oc_plan.argtypes = [ctypes.POINTER(OcContext), ctypes.POINTER(OcParams), _f32p, _f32p, ctypes.c_int,
+                          _f32p, _f32p, ctypes.POINTER(OcOutputs)]
+    L.oc_plan.restype = ctypes.c_int
+    L.oc_trace_path.argtypes = [_f32p, _f32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, _i32p, _f32p, ctypes.c_int]
+    L.oc_trace_path.restype = ctypes.c_int
+    return L
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -80,24 +111,7 @@ def lib():
             from oracle import build as _b  # noqa: PLC0415
 
             _b.build()
-        L = ctypes.CDLL(str(LIB_PATH))
-        L.oc_fk.argtypes = [ctypes.POINTER(OcContext), _f32p, _f32p]
-        L.oc_in_collision.argtypes = [ctypes.POINTER(OcContext), _f32p]
-        L.oc_in_collision.restype = ctypes.c_int
-        L.oc_in_collision_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)]
-        L.oc_min_distance_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, _f32p]
-        L.oc_sphere_positions.argtypes = [ctypes.POINTER(OcContext), _f32p, _f32p]
-        L.oc_halton.argtypes = [ctypes.c_int, ctypes.c_int]
-        L.oc_halton.restype = ctypes.c_float
-        L.oc_prime_order.argtypes = [ctypes.c_int, ctypes.c_uint, _i32p]
-        L.oc_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
-        L.oc_nearest.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int, _i32p, _f32p]
-        L.oc_plan.argtypes = [ctypes.POINTER(OcContext), ctypes.POINTER(OcParams), _f32p, _f32p, ctypes.c_int,
-                              _f32p, _f32p, ctypes.POINTER(OcOutputs)]
-        L.oc_plan.restype = ctypes.c_int
-        L.oc_trace_path.argtypes = [_f32p, _f32p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, _i32p, _f32p, ctypes.c_int]
-        L.oc_trace_path.restype = ctypes.c_int
-        _LIB = L
+        _LIB = bind(ctypes.CDLL(str(LIB_PATH)))
     return _LIB
 
 
@@ -163,10 +177,10 @@ def sphere_positions(ctx: Context, q) -> np.ndarray:
     return out
 
 
-def in_collision(ctx: Context, qs) -> np.ndarray:
+def in_collision(ctx: Context, qs, L=None) -> np.ndarray:
     qs = np.ascontiguousarray(np.asarray(qs, np.float32).reshape(-1, ctx.dim))
     out = np.zeros(qs.shape[0], np.uint8)
-    lib().oc_in_collision_batch(ctypes.byref(ctx.c), _fp(qs), qs.shape[0], out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    (L or lib()).oc_in_collision_batch(ctypes.byref(ctx.c), _fp(qs), qs.shape[0], out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
     return out.astype(bool)
 
 
@@ -198,9 +212,54 @@ def nearest(tree: np.ndarray, size: int, queries) -> tuple[np.ndarray, np.ndarra
     return idx, dsq
 
 
+# ---- planner rules (prrtc_planner.cu:46-83, 790-824, 864-876, 975-993); L: a variant library
+def dd_shrink(r: float, alpha: float, r0: float, rmin: float, L=None) -> float:
+    v = ctypes.c_float(r)
+    (L or lib()).oc_dd_shrink(ctypes.byref(v), alpha, r0, rmin)
+    return v.value
+
+
+def dd_grow(r: float, alpha: float, L=None) -> float:
+    v = ctypes.c_float(r)
+    (L or lib()).oc_dd_grow(ctypes.byref(v), alpha)
+    return v.value
+
+
+def dd_skips(radius: float, nd: float, L=None) -> bool:
+    return bool((L or lib()).oc_dd_skips(radius, nd))
+
+
+def balance_sides(mode: int, rnd: int, S: int, size_a: int, size_b: int, tree_ratio: float, vamp=None, L=None):
+    """Tree of every explorer 0..S-1 in one round; ``vamp`` (mode-2 state per explorer) is
+    updated in place."""
+    vamp = np.zeros(S, np.int32) if vamp is None else vamp
+    out = []
+    for e in range(S):
+        v = ctypes.c_int(int(vamp[e]))
+        out.append((L or lib()).oc_balance_side(mode, rnd, e, S, size_a, size_b, tree_ratio, ctypes.byref(v)))
+        vamp[e] = v.value
+    return np.array(out, np.int32)
+
+
+def steer(base, sample, nd: float, step: float, L=None) -> np.ndarray:
+    b = np.ascontiguousarray(np.asarray(base, np.float32).reshape(-1))
+    s = np.ascontiguousarray(np.asarray(sample, np.float32).reshape(-1))
+    out = np.zeros_like(b)
+    (L or lib()).oc_steer(_fp(b), _fp(s), b.shape[0], nd, step, _fp(out))
+    return out
+
+
+def connect_steps(frm, to, dist: float, step: float, L=None) -> tuple[int, np.ndarray]:
+    f = np.ascontiguousarray(np.asarray(frm, np.float32).reshape(-1))
+    t = np.ascontiguousarray(np.asarray(to, np.float32).reshape(-1))
+    vec = np.zeros_like(f)
+    n = (L or lib()).oc_connect_steps(_fp(f), _fp(t), f.shape[0], dist, step, _fp(vec))
+    return int(n), vec
+
+
 def plan(ctx: Context, start, goals, *, max_iterations=10_000, step_size=0.5, num_new_samples=128,
          granularity=16, max_nodes=16384, balance_mode=1, tree_ratio=0.5, dynamic_domain=True,
-         dd_alpha=1e-4, dd_radius=4.0, dd_min_radius=1.0, min_vals=None, max_vals=None, seed=0) -> dict:
+         dd_alpha=1e-4, dd_radius=4.0, dd_min_radius=1.0, min_vals=None, max_vals=None, seed=0, L=None) -> dict:
     """Run the lockstep planner; returns trees, parents, radii, sizes, iterations, conn, status."""
     dim = ctx.dim
     start = np.ascontiguousarray(np.asarray(start, np.float32).reshape(dim))
@@ -218,25 +277,25 @@ def plan(ctx: Context, start, goals, *, max_iterations=10_000, step_size=0.5, nu
                  1 if dynamic_domain else 0, float(dd_alpha), float(dd_radius), float(dd_min_radius), dim, N,
                  int(granularity), ctypes.c_uint(seed).value)
     o = OcOutputs(_fp(ta), _fp(tb), _ip(pa), _ip(pb), _fp(ra), _fp(rb))
-    rc = lib().oc_plan(ctypes.byref(ctx.c), ctypes.byref(p), _fp(start), _fp(goals), goals.shape[0], _fp(lo),
-                       _fp(hi), ctypes.byref(o))
+    rc = (L or lib()).oc_plan(ctypes.byref(ctx.c), ctypes.byref(p), _fp(start), _fp(goals), goals.shape[0],
+                              _fp(lo), _fp(hi), ctypes.byref(o))
     if rc != 0:
         raise ValueError("oracle plan: invalid arguments")
     sa, sb = o.sizes[0], o.sizes[1]
     res = dict(tree_a=ta, tree_b=tb, parents_a=pa, parents_b=pb, radii_a=ra, radii_b=rb, sizes=(sa, sb),
                iterations=o.iterations, conn=tuple(o.conn), status=o.status, fkcc_evals=o.fkcc_evals)
-    res["path"] = trace_path(res) if o.status == STATUS_SOLVED else None
+    res["path"] = trace_path(res, L=L) if o.status == STATUS_SOLVED else None
     return res
 
 
-def trace_path(res: dict, max_len: int | None = None) -> np.ndarray:
+def trace_path(res: dict, max_len: int | None = None, L=None) -> np.ndarray:
     ta, tb = res["tree_a"], res["tree_b"]
     dim, N = ta.shape
     max_len = max_len or (res["sizes"][0] + res["sizes"][1])
     path = np.zeros((max_len, dim), np.float32)
     conn = np.asarray(res["conn"], np.int32)
-    n = lib().oc_trace_path(_fp(ta), _fp(tb), _ip(res["parents_a"]), _ip(res["parents_b"]), N, dim, _ip(conn),
-                            _fp(path), max_len)
+    n = (L or lib()).oc_trace_path(_fp(ta), _fp(tb), _ip(res["parents_a"]), _ip(res["parents_b"]), N, dim,
+                                   _ip(conn), _fp(path), max_len)
     if n < 0:
         raise RuntimeError("path longer than max_len")
     return path[:n]

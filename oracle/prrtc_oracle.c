@@ -63,6 +63,13 @@
 
 #include "prrtc_oracle.h"
 
+/* OC_PLANT selects one deliberately wrong rule (a swapped comparison, a dropped term, ...) for
+ * the tests that show the oracle's rule tests catch such a mistake (tests/test_oracle_rules.py
+ * builds each variant into a temporary directory).  Every real build has OC_PLANT == 0. */
+#ifndef OC_PLANT
+#define OC_PLANT 0
+#endif
+
 /* ---------------------------------------------------------------- sin/cos -------------- */
 /* Cody-Waite reduction by pi/2 (three-part split of pi/2) and Taylor polynomials on
  * [-pi/4, pi/4]; coefficients are (-1)^k/n! rounded to float. */
@@ -247,17 +254,24 @@ void oc_fk(const oc_context* c, const float* q, float* T_out) {
 }
 
 /* ---------------------------------------------------------------- collision ------------ */
-/* Squared-form binary tests: "dist <= 0" is evaluated as d^2 <= (r1+r2)^2 (and the OBB and
- * halfspace analogues), which agrees with the signed-distance form except within rounding of
- * contact. */
+/* Binary tests: the reference's "dist <= 0" (prrtc_planner.cu:250-304) evaluated without square
+ * roots.  A sphere or capsule whose radius sum rr = r + R is >= 0 is hit iff d^2 <= rr^2; with
+ * rr < 0 never (sqrt(d^2) - rr > 0).  A box with sr >= 0 is hit iff the squared outside distance
+ * is <= sr^2 (it is 0 when the centre is inside); with sr < 0 iff the centre lies deeper than
+ * |sr| inside (max_i q_i <= sr, q_i = |l_i| - h_i).  A halfspace iff n.(c - p) <= sr.  These
+ * agree with the signed-distance form except within rounding of contact. */
 static int hit_sphere(float sx, float sy, float sz, float sr, const float* o) {
     const float dx = sx - o[0], dy = sy - o[1], dz = sz - o[2];
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float rr = sr + o[3];
+#if OC_PLANT == 11
     return d2 <= rr * rr;
+#else
+    return rr >= 0.0f && d2 <= rr * rr;
+#endif
 }
 
-static int hit_capsule(float sx, float sy, float sz, float sr, const float* cap) {
+static float capsule_d2(float sx, float sy, float sz, const float* cap) {
     const float ax = cap[3] - cap[0], ay = cap[4] - cap[1], az = cap[5] - cap[2];
     const float l2 = fmaf(az, az, fmaf(ay, ay, ax * ax));
     const float inv = (l2 > 0.0f) ? 1.0f / l2 : 0.0f;
@@ -265,21 +279,30 @@ static int hit_capsule(float sx, float sy, float sz, float sr, const float* cap)
     float t = fmaf(wz, az, fmaf(wy, ay, wx * ax)) * inv;
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     const float dx = fmaf(-t, ax, wx), dy = fmaf(-t, ay, wy), dz = fmaf(-t, az, wz);
-    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float rr = sr + cap[6];
-    return d2 <= rr * rr;
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-static int hit_box(float sx, float sy, float sz, float sr, const float* b) {
+static int hit_capsule(float sx, float sy, float sz, float sr, const float* cap) {
+    const float rr = sr + cap[6];
+    return rr >= 0.0f && capsule_d2(sx, sy, sz, cap) <= rr * rr;
+}
+
+/* q_i = |l_i| - h_i of the centre in the box frame; returns the squared outside distance. */
+static float box_q(float sx, float sy, float sz, const float* b, float* qmax) {
     const float wx = sx - b[0], wy = sy - b[1], wz = sz - b[2];
     const float l0 = fmaf(wz, b[5], fmaf(wy, b[4], wx * b[3]));
     const float l1 = fmaf(wz, b[8], fmaf(wy, b[7], wx * b[6]));
     const float l2 = fmaf(wz, b[11], fmaf(wy, b[10], wx * b[9]));
-    const float m0 = fmaxf(fabsf(l0) - b[12], 0.0f);
-    const float m1 = fmaxf(fabsf(l1) - b[13], 0.0f);
-    const float m2 = fmaxf(fabsf(l2) - b[14], 0.0f);
-    const float d2 = fmaf(m2, m2, fmaf(m1, m1, m0 * m0));
-    return d2 <= sr * sr;
+    const float q0 = fabsf(l0) - b[12], q1 = fabsf(l1) - b[13], q2 = fabsf(l2) - b[14];
+    const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
+    *qmax = fmaxf(q0, fmaxf(q1, q2));
+    return fmaf(m2, m2, fmaf(m1, m1, m0 * m0));
+}
+
+static int hit_box(float sx, float sy, float sz, float sr, const float* b) {
+    float qmax;
+    const float d2 = box_q(sx, sy, sz, b, &qmax);
+    return (sr >= 0.0f) ? (d2 <= sr * sr) : (qmax <= sr);
 }
 
 static int hit_halfspace(float sx, float sy, float sz, float sr, const float* h) {
@@ -288,32 +311,48 @@ static int hit_halfspace(float sx, float sy, float sz, float sr, const float* h)
     return s <= sr;
 }
 
-/* Signed distances (margin mode and the differentiable cost). */
+/* Margin tests: the reference's "dist < margin" (prrtc_planner.cu:203-206, 314-387), also
+ * without square roots.  With L = r + R + margin, a sphere or capsule is near iff L > 0 and
+ * d^2 < L^2; a box (L = sr + margin) iff L > 0 and outside^2 < L^2 when the centre is outside
+ * (outside^2 > 0), else iff min(max_i q_i, 0) < L; a halfspace iff n.(c - p) - sr < margin. */
+static int near_sphere(float sx, float sy, float sz, float sr, const float* o, float margin) {
+    const float dx = sx - o[0], dy = sy - o[1], dz = sz - o[2];
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float L = (sr + o[3]) + margin;
+    return L > 0.0f && d2 < L * L;
+}
+
+static int near_capsule(float sx, float sy, float sz, float sr, const float* cap, float margin) {
+    const float L = (sr + cap[6]) + margin;
+    return L > 0.0f && capsule_d2(sx, sy, sz, cap) < L * L;
+}
+
+static int near_box(float sx, float sy, float sz, float sr, const float* b, float margin) {
+    float qmax;
+    const float d2 = box_q(sx, sy, sz, b, &qmax);
+    const float L = sr + margin;
+    return (d2 > 0.0f) ? (L > 0.0f && d2 < L * L) : (fminf(qmax, 0.0f) < L);
+}
+
+static int near_halfspace(float sx, float sy, float sz, float sr, const float* h, float margin) {
+    const float off = fmaf(h[2], h[5], fmaf(h[1], h[4], h[0] * h[3]));
+    return (fmaf(h[2], sz, fmaf(h[1], sy, h[0] * sx)) - off) - sr < margin;
+}
+
+/* Signed distances (the min-distance value and the differentiable cost). */
 static float dist_sphere(float sx, float sy, float sz, float sr, const float* o) {
     const float dx = sx - o[0], dy = sy - o[1], dz = sz - o[2];
     return sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - (sr + o[3]);
 }
 
 static float dist_capsule(float sx, float sy, float sz, float sr, const float* cap) {
-    const float ax = cap[3] - cap[0], ay = cap[4] - cap[1], az = cap[5] - cap[2];
-    const float l2 = fmaf(az, az, fmaf(ay, ay, ax * ax));
-    const float inv = (l2 > 0.0f) ? 1.0f / l2 : 0.0f;
-    const float wx = sx - cap[0], wy = sy - cap[1], wz = sz - cap[2];
-    float t = fmaf(wz, az, fmaf(wy, ay, wx * ax)) * inv;
-    t = fminf(fmaxf(t, 0.0f), 1.0f);
-    const float dx = fmaf(-t, ax, wx), dy = fmaf(-t, ay, wy), dz = fmaf(-t, az, wz);
-    return sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - (sr + cap[6]);
+    return sqrtf(capsule_d2(sx, sy, sz, cap)) - (sr + cap[6]);
 }
 
 static float dist_box(float sx, float sy, float sz, float sr, const float* b) {
-    const float wx = sx - b[0], wy = sy - b[1], wz = sz - b[2];
-    const float l0 = fmaf(wz, b[5], fmaf(wy, b[4], wx * b[3]));
-    const float l1 = fmaf(wz, b[8], fmaf(wy, b[7], wx * b[6]));
-    const float l2 = fmaf(wz, b[11], fmaf(wy, b[10], wx * b[9]));
-    const float q0 = fabsf(l0) - b[12], q1 = fabsf(l1) - b[13], q2 = fabsf(l2) - b[14];
-    const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
-    const float outside = sqrtf(fmaf(m2, m2, fmaf(m1, m1, m0 * m0)));
-    const float inside = fminf(fmaxf(q0, fmaxf(q1, q2)), 0.0f);
+    float qmax;
+    const float outside = sqrtf(box_q(sx, sy, sz, b, &qmax));
+    const float inside = fminf(qmax, 0.0f);
     return (outside + inside) - sr;
 }
 
@@ -366,11 +405,12 @@ float oc_min_distance(const oc_context* c, const float* q) {
     return best;
 }
 
+/* Collision verdict: binary mode "any pair dist <= 0", margin mode "any pair dist < margin"
+ * (prrtc_planner.cu:203-206, 232-387), each pair with the square-root-free tests above. */
 int oc_in_collision(const oc_context* c, const float* q) {
     if (!oc_enabled(c)) return 0;
-    if (c->collision_mode == 1) {
-        return oc_min_distance(c, q) < c->collision_margin;
-    }
+    const int mg = c->collision_mode == 1;
+    const float m = c->collision_margin;
     oc_tf T[OC_MAX_JOINTS];
     fk_sorted(c, q, T);
     for (int s = 0; s < c->n_spheres; ++s) {
@@ -378,10 +418,22 @@ int oc_in_collision(const oc_context* c, const float* q) {
         float p[3];
         sphere_world(c, T, s, p);
         const float r = c->sphere_radius[s];
-        for (int i = 0; i < c->n_world_spheres; ++i) if (hit_sphere(p[0], p[1], p[2], r, &c->world_spheres[4 * i])) return 1;
-        for (int i = 0; i < c->n_world_capsules; ++i) if (hit_capsule(p[0], p[1], p[2], r, &c->world_capsules[7 * i])) return 1;
-        for (int i = 0; i < c->n_world_boxes; ++i) if (hit_box(p[0], p[1], p[2], r, &c->world_boxes[15 * i])) return 1;
-        for (int i = 0; i < c->n_world_halfspaces; ++i) if (hit_halfspace(p[0], p[1], p[2], r, &c->world_halfspaces[6 * i])) return 1;
+        for (int i = 0; i < c->n_world_spheres; ++i) {
+            const float* o = &c->world_spheres[4 * i];
+            if (mg ? near_sphere(p[0], p[1], p[2], r, o, m) : hit_sphere(p[0], p[1], p[2], r, o)) return 1;
+        }
+        for (int i = 0; i < c->n_world_capsules; ++i) {
+            const float* o = &c->world_capsules[7 * i];
+            if (mg ? near_capsule(p[0], p[1], p[2], r, o, m) : hit_capsule(p[0], p[1], p[2], r, o)) return 1;
+        }
+        for (int i = 0; i < c->n_world_boxes; ++i) {
+            const float* o = &c->world_boxes[15 * i];
+            if (mg ? near_box(p[0], p[1], p[2], r, o, m) : hit_box(p[0], p[1], p[2], r, o)) return 1;
+        }
+        for (int i = 0; i < c->n_world_halfspaces; ++i) {
+            const float* o = &c->world_halfspaces[6 * i];
+            if (mg ? near_halfspace(p[0], p[1], p[2], r, o, m) : hit_halfspace(p[0], p[1], p[2], r, o)) return 1;
+        }
     }
     for (int k = 0; k < c->n_self_pairs; ++k) {
         const int i = c->self_pairs[2 * k], j = c->self_pairs[2 * k + 1];
@@ -391,7 +443,8 @@ int oc_in_collision(const oc_context* c, const float* q) {
         sphere_world(c, T, i, pi);
         sphere_world(c, T, j, pj);
         const float o[4] = {pj[0], pj[1], pj[2], c->sphere_radius[j]};
-        if (hit_sphere(pi[0], pi[1], pi[2], c->sphere_radius[i], o)) return 1;
+        const float ri = c->sphere_radius[i];
+        if (mg ? near_sphere(pi[0], pi[1], pi[2], ri, o, m) : hit_sphere(pi[0], pi[1], pi[2], ri, o)) return 1;
     }
     return 0;
 }
@@ -468,13 +521,107 @@ typedef struct {
     int size;
 } oc_tree;
 
-static void radius_shrink(float* r, float alpha, float r0, float rmin) {
+/* ---------------------------------------------------------------- planner rules -------- */
+/* Dynamic-domain radius of a node whose extension collided (prrtc_planner.cu:46-64): an
+ * unbounded radius (FLT_MAX) becomes dd_radius, any other shrinks by (1 - alpha), not below
+ * dd_min_radius. */
+void oc_dd_shrink(float* r, float alpha, float r0, float rmin) {
     const float old = *r;
+#if OC_PLANT == 4
+    *r = (old == FLT_MAX) ? r0 : fmaxf(old * (1.0f + alpha), rmin);
+#elif OC_PLANT == 5
+    *r = (old == FLT_MAX) ? r0 : fminf(old * (1.0f - alpha), rmin);
+#else
     *r = (old == FLT_MAX) ? r0 : fmaxf(old * (1.0f - alpha), rmin);
+#endif
 }
 
-static void radius_grow(float* r, float alpha) {
+/* Radius of a node that was extended successfully (prrtc_planner.cu:66-83): grows by
+ * (1 + alpha); an unbounded radius stays unbounded. */
+void oc_dd_grow(float* r, float alpha) {
+#if OC_PLANT == 6
+    *r = *r * (1.0f + alpha);
+#else
     if (*r != FLT_MAX) *r = *r * (1.0f + alpha);
+#endif
+}
+
+/* Dynamic-domain skip (prrtc_planner.cu:868): the sample lies outside the nearest node's
+ * radius. */
+int oc_dd_skips(float radius, float nd) {
+#if OC_PLANT == 7
+    return radius <= nd;
+#else
+    return radius < nd;
+#endif
+}
+
+/* Tree that explorer e of S extends in round `round` (prrtc_planner.cu:790-824); *vamp is the
+ * explorer's balance-mode-2 state (0 or 1, 0 before the first round), updated in place.  Round
+ * 1 always splits the explorers in halves (mode 0). */
+int oc_balance_side(int mode, int round, int e, int S, int size_a, int size_b, float tree_ratio, int* vamp) {
+    const int total = size_a + size_b;
+#if OC_PLANT == 8
+    if (mode == 0 || round == 1) return (e <= S / 2) ? 0 : 1;
+#else
+    if (mode == 0 || round == 1) return (e < S / 2) ? 0 : 1;
+#endif
+    if (mode == 1) {
+        const int diff = (size_a >= size_b) ? (size_a - size_b) : (size_b - size_a);
+        const float ratio = (total > 0) ? ((float)size_a / (float)total) : 0.5f;
+#if OC_PLANT == 1
+        if (diff < (int)(1.5f * (float)S)) return (e < (int)((float)S * (1.0f - ratio))) ? 0 : 1;
+        return (ratio > tree_ratio) ? 0 : 1;
+#elif OC_PLANT == 2
+        if (diff < (int)(1.5f * (float)S)) return (e < (int)((float)S * ratio)) ? 0 : 1;
+        return (ratio < tree_ratio) ? 0 : 1;
+#else
+        if (diff < (int)(1.5f * (float)S)) return (e < (int)((float)S * (1.0f - ratio))) ? 0 : 1;
+        return (ratio < tree_ratio) ? 0 : 1;
+#endif
+    }
+    if (mode == 2) {
+        const int o_id = 1 - *vamp;
+        const int t_sz = (*vamp == 0) ? size_a : size_b;
+        const int o_sz = (o_id == 0) ? size_a : size_b;
+        const float ratio = (t_sz > 0) ? fabsf((float)(t_sz - o_sz) / (float)t_sz) : 0.0f;
+#if OC_PLANT == 3
+        if (ratio >= tree_ratio) *vamp = 1 - *vamp;
+#else
+        if (ratio < tree_ratio) *vamp = 1 - *vamp;
+#endif
+        return *vamp;
+    }
+#if OC_PLANT == 9
+    return (size_a < size_b) ? 0 : 1;
+#else
+    return (size_a <= size_b) ? 0 : 1;
+#endif
+}
+
+/* Steer (prrtc_planner.cu:870-876): from base toward sample by at most `step`, where nd is the
+ * base-to-sample distance; a sample within 1e-8 is taken as is. */
+void oc_steer(const float* base, const float* sample, int dim, float nd, float step, float* out) {
+#if OC_PLANT == 10
+    const float sc = (nd > 1e-8f) ? fminf(1.0f, nd / step) : 1.0f;
+#else
+    const float sc = (nd > 1e-8f) ? fminf(1.0f, step / nd) : 1.0f;
+#endif
+    for (int d = 0; d < dim; ++d) out[d] = base[d] + (sample[d] - base[d]) * sc;
+}
+
+/* Connect walk (prrtc_planner.cu:975-993): the number of sub-edges n = ceil(dist / step) (0 when
+ * the step or the distance is within 1e-8) from `from` to `to`, and the sub-edge vector
+ * (to - from) / n (zero when n == 0).  Returns n. */
+int oc_connect_steps(const float* from, const float* to, int dim, float dist, float step, float* vec) {
+    int n = 0;
+#if OC_PLANT == 12
+    if (step > 1e-8f && dist > 1e-8f) n = (int)floorf(dist / step);
+#else
+    if (step > 1e-8f && dist > 1e-8f) n = (int)ceilf(dist / step);
+#endif
+    for (int d = 0; d < dim; ++d) vec[d] = (n > 0) ? (to[d] - from[d]) / (float)n : 0.0f;
+    return n;
 }
 
 static int edge_in_collision(const oc_context* c, int dim, int G, const float* base, const float* delta_or_end,
@@ -536,35 +683,15 @@ int oc_plan(const oc_context* c, const oc_params* p, const float* start, const f
                 const float val = oc_halton(prime[16 * e + (d % 16)], r);
                 sample[d] = min_vals[d] + val * (max_vals[d] - min_vals[d]);
             }
-            int t;
-            const int total = size_a + size_b;
-            if (p->balance_mode == 0 || r == 1) {
-                t = (e < S / 2) ? 0 : 1;
-            } else if (p->balance_mode == 1) {
-                const int diff = (size_a >= size_b) ? (size_a - size_b) : (size_b - size_a);
-                const float ratio = (total > 0) ? ((float)size_a / (float)total) : 0.5f;
-                if (diff < (int)(1.5f * (float)S)) t = (e < (int)((float)S * (1.0f - ratio))) ? 0 : 1;
-                else t = (ratio < p->tree_ratio) ? 0 : 1;
-            } else if (p->balance_mode == 2) {
-                const int o_id = 1 - vamp[e];
-                const int t_sz = (vamp[e] == 0) ? size_a : size_b;
-                const int o_sz = (o_id == 0) ? size_a : size_b;
-                const float ratio = (t_sz > 0) ? fabsf((float)(t_sz - o_sz) / (float)t_sz) : 0.0f;
-                if (ratio < p->tree_ratio) vamp[e] = 1 - vamp[e];
-                t = vamp[e];
-            } else {
-                t = (size_a <= size_b) ? 0 : 1;
-            }
+            const int t = oc_balance_side(p->balance_mode, r, e, S, size_a, size_b, p->tree_ratio, &vamp[e]);
             side[e] = t;
             float bd;
             nn_scan(tr[t].cfg, N, tr[t].size, dim, sample, &nn[e], &bd);
             const float nd = sqrtf(bd);
-            if (p->dynamic_domain && tr[t].radius[nn[e]] < nd) { state[e] = 0; continue; }
-            const float sc = (nd > 1e-8f) ? fminf(1.0f, p->step_size / nd) : 1.0f;
-            for (int d = 0; d < dim; ++d) {
-                const float base = tr[t].cfg[(size_t)d * N + nn[e]];
-                cand[e * dim + d] = base + (sample[d] - base) * sc;
-            }
+            if (p->dynamic_domain && oc_dd_skips(tr[t].radius[nn[e]], nd)) { state[e] = 0; continue; }
+            float base[OC_MAX_DIM];
+            for (int d = 0; d < dim; ++d) base[d] = tr[t].cfg[(size_t)d * N + nn[e]];
+            oc_steer(base, sample, dim, nd, p->step_size, &cand[e * dim]);
             state[e] = 1;
         }
         /* phase 2: extend-edge FK+CC */
@@ -578,7 +705,7 @@ int oc_plan(const oc_context* c, const oc_params* p, const float* start, const f
         /* phase 3: commit.  All dynamic-domain shrinks, then appends (with grows), in explorer order. */
         if (p->dynamic_domain)
             for (int e = 0; e < S; ++e)
-                if (state[e] == 1) radius_shrink(&tr[side[e]].radius[nn[e]], p->dd_alpha, p->dd_radius, p->dd_min_radius);
+                if (state[e] == 1) oc_dd_shrink(&tr[side[e]].radius[nn[e]], p->dd_alpha, p->dd_radius, p->dd_min_radius);
         for (int e = 0; e < S; ++e) {
             newi[e] = -1;
             if (state[e] != 2) continue;
@@ -588,7 +715,7 @@ int oc_plan(const oc_context* c, const oc_params* p, const float* start, const f
             for (int d = 0; d < dim; ++d) T->cfg[(size_t)d * N + idx] = cand[e * dim + d];
             T->parent[idx] = nn[e];
             T->radius[idx] = FLT_MAX;
-            if (p->dynamic_domain) radius_grow(&T->radius[nn[e]], p->dd_alpha);
+            if (p->dynamic_domain) oc_dd_grow(&T->radius[nn[e]], p->dd_alpha);
             newi[e] = idx;
         }
         if (status != 0) break;
@@ -600,12 +727,9 @@ int oc_plan(const oc_context* c, const oc_params* p, const float* start, const f
             const oc_tree* O = &tr[1 - side[e]];
             float bd;
             nn_scan(O->cfg, N, side[e] ? size_a : size_b, dim, &cand[e * dim], &onn[e], &bd);
-            const float cd = sqrtf(bd);
-            int n_ext = 0;
-            if (p->step_size > 1e-8f && cd > 1e-8f) n_ext = (int)ceilf(cd / p->step_size);
-            next[e] = n_ext;
-            for (int d = 0; d < dim; ++d)
-                vec[e * dim + d] = (n_ext > 0) ? (O->cfg[(size_t)d * N + onn[e]] - cand[e * dim + d]) / (float)n_ext : 0.0f;
+            float target[OC_MAX_DIM];
+            for (int d = 0; d < dim; ++d) target[d] = O->cfg[(size_t)d * N + onn[e]];
+            next[e] = oc_connect_steps(&cand[e * dim], target, dim, sqrtf(bd), p->step_size, &vec[e * dim]);
         }
         /* phase 5: connect sub-edges, first failing one ends the walk */
         for (int e = 0; e < S; ++e) {

@@ -79,6 +79,12 @@ void oc_sphere_positions(const oc_context* c, const float* q, float* out);
 float oc_halton(int base, int index);
 void oc_prime_order(int explorer, unsigned seed, int* out16);
 void oc_nearest(const float* tree, int max_nodes, int size, int dim, const float* qs, int nq, int* idx, float* dsq);
+void oc_dd_shrink(float* r, float alpha, float r0, float rmin);
+void oc_dd_grow(float* r, float alpha);
+int oc_dd_skips(float radius, float nd);
+int oc_balance_side(int mode, int round, int e, int S, int size_a, int size_b, float tree_ratio, int* vamp);
+void oc_steer(const float* base, const float* sample, int dim, float nd, float step, float* out);
+int oc_connect_steps(const float* from, const float* to, int dim, float dist, float step, float* vec);
 int oc_plan(const oc_context* c, const oc_params* p, const float* start, const float* goals, int num_goals,
             const float* min_vals, const float* max_vals, oc_outputs* out);
 int oc_trace_path(const float* tree_a, const float* tree_b, const int* par_a, const int* par_b, int max_nodes,
