@@ -177,9 +177,11 @@ int prrtc_planner_batch(prrtc_stream_t stream, int32_t batch, const float* start
 
 /* Collision verdict of n configs [n, dim] against one robot + world (no reference FFI target;
  * it is the planner's prrtc_config_in_collision, prrtc_planner.cu:501-531, exposed for parity
- * and throughput).  out_hit [n] uint8 (may be NULL), out_min_dist [n] float (may be NULL; only
- * computed in collision_mode 1 or when out_hit is NULL).  work_flops (may be NULL): a device
- * uint64 that the call atomically increments by the algorithmic flops of the checking it did. */
+ * and throughput).  out_hit [n] uint8 (may be NULL): the verdict of collision_mode (binary: any
+ * pair distance <= 0; 1: any pair distance < collision_margin).  out_min_dist [n] float (may be
+ * NULL): the smallest signed distance over every pair (a full sweep; square roots within 1 ulp).
+ * work_flops (may be NULL): a device uint64 that the call atomically increments by the
+ * algorithmic flops of the checking it did. */
 int prrtc_check_configs(prrtc_stream_t stream, const float* configs, int32_t n, int32_t dim, const prrtc_robot* robot,
                         const prrtc_world* world, int32_t collision_mode, float collision_margin, uint8_t* out_hit,
                         float* out_min_dist, const prrtc_launch* launch, uint64_t* work_flops);

@@ -156,11 +156,16 @@ struct Model {
 };
 
 // ---------------------------------------------------------------------------------- tests
+// Binary tests: the reference's "dist <= 0" (prrtc_planner.cu:250-304) without square roots.  A
+// sphere or capsule whose radius sum rr = r + R is >= 0 is hit iff d^2 <= rr^2, never when
+// rr < 0 (sqrt(d^2) - rr > 0); a box with sr >= 0 iff the squared outside distance is <= sr^2
+// (0 when the centre is inside), with sr < 0 iff the centre lies deeper than |sr| inside
+// (max q <= sr); a halfspace iff n.(c - p) <= sr.
 __device__ __forceinline__ bool hit_sphere(float sx, float sy, float sz, float sr, const float4 o) {
     const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     const float rr = sr + o.w;
-    return d2 <= rr * rr;
+    return rr >= 0.0f && d2 <= rr * rr;
 }
 
 __device__ __forceinline__ float capsule_d2(float sx, float sy, float sz, const CapsuleL& c) {
@@ -173,7 +178,7 @@ __device__ __forceinline__ float capsule_d2(float sx, float sy, float sz, const 
 
 __device__ __forceinline__ bool hit_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c) {
     const float rr = sr + c.r;
-    return capsule_d2(sx, sy, sz, c) <= rr * rr;
+    return rr >= 0.0f && capsule_d2(sx, sy, sz, c) <= rr * rr;
 }
 
 __device__ __forceinline__ void box_q(float sx, float sy, float sz, const BoxL& b, float& q0, float& q1, float& q2) {
@@ -186,12 +191,19 @@ __device__ __forceinline__ void box_q(float sx, float sy, float sz, const BoxL& 
     q2 = fabsf(l2) - b.h2;
 }
 
-__device__ __forceinline__ bool hit_box(float sx, float sy, float sz, float sr, const BoxL& b) {
+// Squared outside distance of a box; qmax = the largest q (<= 0 inside).
+__device__ __forceinline__ float box_d2(float sx, float sy, float sz, const BoxL& b, float& qmax) {
     float q0, q1, q2;
     box_q(sx, sy, sz, b, q0, q1, q2);
     const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
-    const float d2 = __builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0));
-    return d2 <= sr * sr;
+    qmax = fmaxf(q0, fmaxf(q1, q2));
+    return __builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0));
+}
+
+__device__ __forceinline__ bool hit_box(float sx, float sy, float sz, float sr, const BoxL& b) {
+    float qmax;
+    const float d2 = box_d2(sx, sy, sz, b, qmax);
+    return (sr >= 0.0f) ? (d2 <= sr * sr) : (qmax <= sr);
 }
 
 __device__ __forceinline__ bool hit_halfspace(float sx, float sy, float sz, float sr, const float4 h) {
@@ -199,21 +211,51 @@ __device__ __forceinline__ bool hit_halfspace(float sx, float sy, float sz, floa
     return s <= sr;
 }
 
+// Margin tests: the reference's "dist < margin" (prrtc_planner.cu:203-206, 314-387), also without
+// square roots.  With L = r + R + margin, a sphere or capsule is near iff L > 0 and d^2 < L^2; a
+// box (L = sr + margin) iff L > 0 and outside^2 < L^2 when the centre is outside (outside^2 > 0),
+// else iff min(max q, 0) < L; a halfspace iff n.(c - p) - sr < margin.
+__device__ __forceinline__ bool near_sphere(float sx, float sy, float sz, float sr, const float4 o, float margin) {
+    const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float L = (sr + o.w) + margin;
+    return L > 0.0f && d2 < L * L;
+}
+
+__device__ __forceinline__ bool near_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c, float margin) {
+    const float L = (sr + c.r) + margin;
+    return L > 0.0f && capsule_d2(sx, sy, sz, c) < L * L;
+}
+
+__device__ __forceinline__ bool near_box(float sx, float sy, float sz, float sr, const BoxL& b, float margin) {
+    float qmax;
+    const float d2 = box_d2(sx, sy, sz, b, qmax);
+    const float L = sr + margin;
+    return (d2 > 0.0f) ? (L > 0.0f && d2 < L * L) : (fminf(qmax, 0.0f) < L);
+}
+
+__device__ __forceinline__ bool near_halfspace(float sx, float sy, float sz, float sr, const float4 h, float margin) {
+    return (__builtin_fmaf(h.z, sz, __builtin_fmaf(h.y, sy, h.x * sx)) - h.w) - sr < margin;
+}
+
+// Signed distances (the minimum-distance value of the flat check).  The square root is the
+// hardware's (v_sqrt_f32, within 1 ulp), not the correctly rounded one: no verdict depends on
+// these values.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ float dist_sphere(float sx, float sy, float sz, float sr, const float4 o) {
     const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
-    return __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) - (sr + o.w);
+    return fast_sqrt(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) - (sr + o.w);
 }
 
 __device__ __forceinline__ float dist_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c) {
-    return __builtin_sqrtf(capsule_d2(sx, sy, sz, c)) - (sr + c.r);
+    return fast_sqrt(capsule_d2(sx, sy, sz, c)) - (sr + c.r);
 }
 
 __device__ __forceinline__ float dist_box(float sx, float sy, float sz, float sr, const BoxL& b) {
-    float q0, q1, q2;
-    box_q(sx, sy, sz, b, q0, q1, q2);
-    const float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
-    const float outside = __builtin_sqrtf(__builtin_fmaf(m2, m2, __builtin_fmaf(m1, m1, m0 * m0)));
-    const float inside = fminf(fmaxf(q0, fmaxf(q1, q2)), 0.0f);
+    float qmax;
+    const float outside = fast_sqrt(box_d2(sx, sy, sz, b, qmax));
+    const float inside = fminf(qmax, 0.0f);
     return (outside + inside) - sr;
 }
 
@@ -501,23 +543,23 @@ template <bool MARGIN>
 __device__ __forceinline__ bool fine_test(const Model& M, int o, const float4 q, unsigned& fl) {
     if (o < M.n_wh) {
         fl += kFlopsSH;
-        return MARGIN ? dist_halfspace(q.x, q.y, q.z, q.w, M.whs[o]) < M.margin
+        return MARGIN ? near_halfspace(q.x, q.y, q.z, q.w, M.whs[o], M.margin)
                       : hit_halfspace(q.x, q.y, q.z, q.w, M.whs[o]);
     }
     o -= M.n_wh;
     if (o < M.n_wb) {
         fl += kFlopsSB;
-        return MARGIN ? dist_box(q.x, q.y, q.z, q.w, M.wbox[o]) < M.margin : hit_box(q.x, q.y, q.z, q.w, M.wbox[o]);
+        return MARGIN ? near_box(q.x, q.y, q.z, q.w, M.wbox[o], M.margin) : hit_box(q.x, q.y, q.z, q.w, M.wbox[o]);
     }
     o -= M.n_wb;
     if (o < M.n_wc) {
         fl += kFlopsSC;
-        return MARGIN ? dist_capsule(q.x, q.y, q.z, q.w, M.wcap[o]) < M.margin
+        return MARGIN ? near_capsule(q.x, q.y, q.z, q.w, M.wcap[o], M.margin)
                       : hit_capsule(q.x, q.y, q.z, q.w, M.wcap[o]);
     }
     o -= M.n_wc;
     fl += kFlopsSS;
-    return MARGIN ? dist_sphere(q.x, q.y, q.z, q.w, M.wsph[o]) < M.margin : hit_sphere(q.x, q.y, q.z, q.w, M.wsph[o]);
+    return MARGIN ? near_sphere(q.x, q.y, q.z, q.w, M.wsph[o], M.margin) : hit_sphere(q.x, q.y, q.z, q.w, M.wsph[o]);
 }
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
@@ -547,17 +589,22 @@ __device__ __forceinline__ bool self_pairs_2(const Model& M, const float4* FP, i
     bool h = false;
     if (k < M.n_pairs) {
         const float4 a = FP[M.pairs[k] & 0xffffu], b = FP[M.pairs[k] >> 16];
-        h = MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+        h = MARGIN ? near_sphere(a.x, a.y, a.z, a.w, b, M.margin) : hit_sphere(a.x, a.y, a.z, a.w, b);
     }
     if (k2 < M.n_pairs) {
         const float4 a = FP[M.pairs[k2] & 0xffffu], b = FP[M.pairs[k2] >> 16];
-        h |= MARGIN ? dist_sphere(a.x, a.y, a.z, a.w, b) < M.margin : hit_sphere(a.x, a.y, a.z, a.w, b);
+        h |= MARGIN ? near_sphere(a.x, a.y, a.z, a.w, b, M.margin) : hit_sphere(a.x, a.y, a.z, a.w, b);
     }
     return h;
 }
 
+// The two checks below are kept out of line (noinline): inlined into the planner's round loop
+// they push its 16-dimension team kernel past 256 VGPRs, the compiler then outlines the whole
+// round loop with a copy of the kernel arguments in private memory (about 1.5 KB per lane), and a
+// cooperative launch that keeps every work-group resident needs that much scratch for every lane
+// of the GPU at once.  Out of line, each check has its own small frame.
 template <bool MARGIN>
-__device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
+__device__ __attribute__((noinline)) bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
     char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
     float4* FP = reinterpret_cast<float4*>(area);          // robot spheres, world frame
@@ -718,7 +765,7 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
 // step, stopping at the first step with a hit), and, if none hits the world, the self pairs of
 // each configuration in order.  Whether any configuration collides is wave_culled's verdict.
 template <bool MARGIN>
-__device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
+__device__ __attribute__((noinline)) int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
     char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
     float4* FP = reinterpret_cast<float4*>(area);          // robot spheres (per configuration)

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(1024) void check_kernel(CheckArgs a) {
                 const float md = wave_check<true>(M, W.T + c, kFBP, flops);
                 if (lane == 0) a.out_min[idx] = md;
                 if (a.out_hit) {
-                    const bool h = (a.mode == 1) ? (M.enabled && md < a.margin) : wave_in_collision(M, W.T + c, kFBP, flops);
+                    const bool h = wave_in_collision(M, W.T + c, kFBP, flops);
                     if (lane == 0) a.out_hit[idx] = h ? 1 : 0;
                 }
             } else {

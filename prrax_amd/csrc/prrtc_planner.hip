@@ -1265,6 +1265,22 @@ size_t planner_lds(const ModelLayout& ml, int S, int dim, PlanArgs& a) {
     return o;
 }
 
+// Private memory (scratch) of the planner kernels per lane.  The team pass is a cooperative launch
+// that keeps every resident work-group alive at once, so its scratch is needed for every lane of
+// the GPU together: a launch needing more than kMaxPassScratch bytes in total is refused instead
+// of being left to the runtime's scratch allocation.
+constexpr size_t kMaxPassScratch = size_t(64) << 20;
+
+template <int MAXD>
+size_t pass_scratch_per_lane() {
+    hipFuncAttributes f1, f2;
+    std::memset(&f1, 0, sizeof(f1));
+    std::memset(&f2, 0, sizeof(f2));
+    if (hipFuncGetAttributes(&f1, reinterpret_cast<const void*>(&plan_pass1<MAXD>)) != hipSuccess) return 0;
+    if (hipFuncGetAttributes(&f2, reinterpret_cast<const void*>(&plan_pass2<MAXD>)) != hipSuccess) return 0;
+    return std::max(f1.localSizeBytes, f2.localSizeBytes);
+}
+
 template <int MAXD>
 int occupancy(int nt, size_t lds) {
     int b1 = 0, b2 = 0;
@@ -1393,6 +1409,12 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
                                      (static_cast<unsigned long long>(nt) << 16) |
                                      (static_cast<unsigned long long>(a.pass1_rounds & 0xffff) << 32) |
                                      (static_cast<unsigned long long>(a.wmax & 0xffff) << 48);
+    const size_t scratch_lane = small ? pass_scratch_per_lane<8>() : pass_scratch_per_lane<16>();
+    if (scratch_lane * static_cast<size_t>(nt) * static_cast<size_t>(a.slots) > kMaxPassScratch)
+        return fail_with(PRRTC_E_HIP, "planner kernels use " + std::to_string(scratch_lane) +
+                                          " bytes of private memory per lane: a cooperative launch of " +
+                                          std::to_string(a.slots) + " work-groups would need more than " +
+                                          std::to_string(kMaxPassScratch >> 20) + " MiB of scratch at once");
     hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, shape);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));

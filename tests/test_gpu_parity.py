@@ -7,6 +7,9 @@ arithmetic (FK poses, tree configurations, distances) must be identical.
 
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 import torch
@@ -25,6 +28,13 @@ def say(msg):
     print(msg, flush=True)
 
 
+def _oracle_parallel(fn, qs, chunk=25_000):
+    """fn over row chunks of qs on the host's cores (the oracle's ctypes calls release the GIL)."""
+    parts = [qs[i:i + chunk] for i in range(0, qs.shape[0], chunk)]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return np.concatenate(list(ex.map(fn, parts)))
+
+
 def _robot_scenes():
     return {"c1": scenes.scene_c1(), "c2": scenes.scene_c2(), "c4": scenes.scene_c4()}
 
@@ -39,11 +49,16 @@ def test_fk_bit_exact(name):
     say(f"fk {name} n=4096")
     got = forward_kinematics(cc, q).cpu().numpy()
     ctx = O.Context(cc, dim)
-    want = np.stack([O.fk(ctx, qi) for qi in q[:512]])
-    np.testing.assert_array_equal(got[:512].view(np.uint32), want.view(np.uint32))
+    want = np.stack([O.fk(ctx, qi) for qi in q])
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("name,n", [("c1", 200_000), ("c2", 50_000), ("c4", 20_000)])
+# Minimum-distance values use the hardware square root (within 1 ulp); the distances in these
+# scenes are below 4 m, where 1 ulp is at most 4.8e-7 m.
+MIN_DIST_ATOL = 1e-6
+
+
+@pytest.mark.parametrize("name,n", [("c1", 200_000), ("c2", 1_000_000), ("c4", 20_000)])
 @pytest.mark.parametrize("mode", ["binary", "differentiable"])
 def test_collision_bit_exact(name, n, mode):
     sc = _robot_scenes()[name]
@@ -58,16 +73,16 @@ def test_collision_bit_exact(name, n, mode):
     hit = chk.in_collision(q).cpu().numpy()
     mind = chk.min_distance(q[:5000]).cpu().numpy()
     ctx = O.Context(cc, dim, collision_mode=1 if mode == "differentiable" else 0, collision_margin=0.005)
-    want = O.in_collision(ctx, q)
+    want = _oracle_parallel(lambda part: O.in_collision(ctx, part), q)
     np.testing.assert_array_equal(hit, want)
-    np.testing.assert_array_equal(mind.view(np.uint32), O.min_distance(ctx, q[:5000]).view(np.uint32))
+    np.testing.assert_allclose(mind, O.min_distance(ctx, q[:5000]), rtol=0, atol=MIN_DIST_ATOL)
     assert 0.05 < hit.mean() < 0.999  # both outcomes well represented
 
 
 @pytest.mark.parametrize("mode", ["binary", "differentiable", "binary_coarse", "binary_coarse_coop"])
 def test_collision_degenerate_geometry_bit_exact(mode):
-    """Verdicts on worlds the coarse culling must not prune: negative radii (squared-form
-    tests), non-orthonormal boxes, non-unit halfspace normals, degenerate capsules; a tree robot
+    """Verdicts on worlds the coarse culling must not prune: negative radii (no contact when a
+    radius sum is negative), non-orthonormal boxes, non-unit halfspace normals, degenerate capsules; a tree robot
     with prismatic, screw and mimic joints; configurations concentrated near contact."""
     from test_oracle_fk_collision import random_tree_robot
 
@@ -180,6 +195,23 @@ def test_planner_edge_shapes_bit_exact(S, G, N, team):
     res = prrtc_plan_batch(start[None], [goals], collision_context=cc, team_max=team, pass1_rounds=0, **kw)[0]
     ora = O.plan(O.Context(cc, 7), start, goals, **kw)
     compare_plan(res, ora, f"S{S} G{G} N{N}")
+
+
+def test_planner_c2_bit_exact():
+    """The large scene (64 spheres, 200 self pairs, 200 spheres, 16 boxes, 8 capsules, floor; 4
+    goals, S = 512, max_nodes 65536): three seeded queries in one batch against the oracle."""
+    sc = scenes.scene_c2()
+    cc = sc.context()
+    qs = [query(sc, 2, qi, n_goals=4) for qi in range(3)]
+    kw = dict(num_new_samples=512, max_nodes=65536, max_iterations=10_000, min_vals=sc.robot.lower,
+              max_vals=sc.robot.upper)
+    say("plan c2 x3 batch vs oracle")
+    res = prrtc_plan_batch(np.stack([s for s, _ in qs]), np.stack([g for _, g in qs]), collision_context=cc, **kw)
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        oras = list(ex.map(lambda sg: O.plan(O.Context(cc, 7), sg[0], sg[1], **kw), qs))
+    for qi, (r, ora) in enumerate(zip(res, oras)):
+        compare_plan(r, ora, f"c2 q{qi}")
+        assert r.solved
 
 
 def test_planner_failure_modes_bit_exact():
