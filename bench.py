@@ -1,10 +1,15 @@
 """Planner benchmark: batched persistent launch vs per-query launches on 1024 uneven 7-DOF queries.
 
 A step is one batched plan of the whole 1024-query workload (one launch of the persistent
-planner).  ``value`` is plans/s of the batched planner with every input already in HBM; the
-same queries are also run as 1024 sequential single-query launches (each followed by a stream
-synchronisation, as a per-query caller would) and both solve rates are reported.  FK+CC
-throughput is measured on 10^6 configurations against the large-scene world.
+planner: reset, per-query pass, cooperative team pass).  ``value`` is plans/s of the batched
+planner with every input already in HBM.  The autotuner picks samples per round, waves per query
+and the FK/check tile on a tail-seeded sample first.  Outside the timed region every returned
+path is re-validated by the host FP64 checker at t = k/16, the same queries run as 1024
+sequential single-query launches (each followed by a synchronisation, as a per-query caller
+would), and the other legs run: FK+CC throughput on 10^6 configurations against the large-scene
+world, the large scene itself (100 seeded queries, S = 512, 65536 nodes), a 4096-query batch,
+the 14-DOF dual arm, path refinement, and the CPU baselines (the reference's host RRT-Connect
+loop restated in NumPy, on the host's cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--queries 1024] [--no-cpu]
 """
@@ -13,7 +18,10 @@ from __future__ import annotations
 
 import argparse
 import json
+import multiprocessing
 import os
+import platform
+import subprocess
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -25,9 +33,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (datasheet)
 PEAK_HBM_GBS = 8000.0
-# HBM bytes of one batched step of the default 1024-query workload (plan_pass1 + plan_pass2),
-# from rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) and WRITE_SIZE (x 1.0).
-TRAFFIC_BYTES_PER_STEP = 16.82e9
+# HBM bytes of one batched step of the default 1024-query workload (plan_pass1 + plan_pass2) at
+# the autotuned shape: rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) plus
+# WRITE_SIZE (x 1.0), from the profile named below (measured by its own runs, not by this one).
+TRAFFIC = {"bytes_per_step": None, "source": None}
 
 METRIC = "plans/sec + p50 ms/plan, batched vs per-query at equal solve rate; FK+CC cfg/s"
 
@@ -36,6 +45,32 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_workers() -> int:
+    """Host threads this run may use: the process's CPU affinity, at most 16 (one GPU's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def host_info() -> dict:
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, subprocess.SubprocessError, ValueError):
+        nproc = os.cpu_count() or 0
+    return {"nproc": nproc, "cpu_model": model, "affinity": cpu_workers()}
+
+
+# ------------------------------------------------------------------------------ workloads
 def build_c3(n_queries: int, checker_factory):
     """Robot, per-query worlds/goals/starts/max_nodes of the uneven 1024-query workload."""
     from prrax_amd import scenes
@@ -56,41 +91,85 @@ def build_c3(n_queries: int, checker_factory):
     return robot, worlds, starts, goals, nodes
 
 
-def _cpu_plan(args):
-    cc, start, goals, kw = args
-    from oracle import oracle as O
+def gpu_free_checker(dim: int):
+    from prrax_amd.prrtc import CollisionChecker
 
-    r = O.plan(O.Context(cc, 7), start, goals, **kw)
-    return r["status"] == O.STATUS_SOLVED
+    def factory(cc):
+        return CollisionChecker(cc, dim, collision_checker="differentiable", collision_margin=1e-5).is_free
 
-
-def cpu_baseline(worlds, starts, goals, nodes, kw, n_sample: int, workers: int):
-    jobs = [(worlds[i], starts[i], goals[i], dict(kw, max_nodes=nodes[i])) for i in range(n_sample)]
-    t0 = time.perf_counter()
-    # ctypes releases the GIL for the duration of each oracle call, so threads run in parallel.
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        solved = list(ex.map(_cpu_plan, jobs))
-    dt = time.perf_counter() - t0
-    return {"value": round(n_sample / dt, 3), "unit": "plans/s", "cores": workers, "kind": "port",
-            "sample": f"first {n_sample} of the {len(worlds)} queries, C oracle planner (same lockstep algorithm), "
-                      f"one query per thread",
-            "solve_rate": round(float(np.mean(solved)), 4), "wall_s": round(dt, 2)}
+    return factory
 
 
-def copy_calibration(mib: int):
-    """One plain copy of ``mib`` MiB: its FETCH_SIZE / WRITE_SIZE readings calibrate the HBM counters."""
+def time_singles(singles, label: str):
     import torch
 
-    n = mib * (1 << 20) // 4
-    src = torch.arange(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
+    singles[0].enqueue()
     torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    log(f"copy calibration: {mib} MiB read + {mib} MiB written")
+    times = []
+    for k, sp in enumerate(singles):
+        t = time.perf_counter()
+        sp.enqueue()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        if (k + 1) % 256 == 0:
+            log(f"{label}: {k + 1}/{len(singles)} single-query launches, {sum(times):.2f}s")
+    solved = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
+    return np.array(times), solved
 
 
-def scale_leg(n_queries: int, checker_factory, kw: dict, launch: dict):
+def validate_leg(results, worlds) -> dict:
+    """Every returned path re-validated by the host FP64 checker at t = k/16 of each edge."""
+    from prrax_amd.validate import validate_paths
+
+    t0 = time.perf_counter()
+    paths = [r.path if r.solved else None for r in results]
+    ok = validate_paths(worlds, paths, resolution=16, tol=1e-5, workers=cpu_workers())
+    solved = int(sum(r.solved for r in results))
+    out = {"solved": solved, "validated": int(ok.sum()), "checker": "host FP64 signed distances, t = k/16 of every "
+           "edge, contact tolerance 1e-5 m", "wall_s": round(time.perf_counter() - t0, 2)}
+    log(f"validation: {out}")
+    return out
+
+
+def c2_leg(n_queries: int, kw_base: dict):
+    """The large scene (64 spheres, 200 self pairs, 200 spheres, 16 boxes, 8 capsules, floor; 4
+    goals, S = 512, 65536 nodes): one batched launch of the seeded queries and per-query launches."""
+    import torch
+
+    from prrax_amd import prepare_batch, scenes
+
+    sc = scenes.scene_c2()
+    cc = sc.context()
+    free = gpu_free_checker(7)(cc)
+    starts, goals = [], []
+    for qi in range(n_queries):
+        sg = scenes.sample_free(np.random.default_rng(scenes.query_seed(2, qi)), sc.robot, 5, free)
+        starts.append(sg[0])
+        goals.append(sg[1:])
+    kw = dict(kw_base, num_new_samples=512, max_nodes=65536, min_vals=sc.robot.lower, max_vals=sc.robot.upper)
+    pb = prepare_batch(np.stack(starts), np.stack(goals), collision_context=cc, **kw)
+    pb.enqueue()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pb.enqueue()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    h = pb.bufs.host_summary()
+    singles = [prepare_batch(starts[i][None], goals[i][None], collision_context=cc, **kw) for i in range(n_queries)]
+    times, solved_pq = time_singles(singles, "large scene")
+    out = {"workload": f"{n_queries} seeded queries, large scene (64 spheres, 200 self pairs, 200 spheres, 16 boxes, "
+                       "8 capsules, floor), 4 goals, S 512, max_nodes 65536",
+           "batched_plans_per_s": round(n_queries / dt, 1), "batched_ms": round(1000 * dt, 2),
+           "p50_ms_per_plan_device": round(float(np.median(h["kernel_time_ms"])), 3),
+           "solve_rate": round(float(np.mean(h["solved"] == 1)), 4),
+           "per_query_plans_per_s": round(n_queries / times.sum(), 1),
+           "per_query_p50_ms": round(1000 * float(np.median(times)), 3),
+           "identical_solved_flags": bool(np.array_equal(h["solved"] == 1, solved_pq))}
+    log(f"large scene: {out}")
+    return out
+
+
+def scale_leg(n_queries: int, kw: dict, launch: dict):
     """The same uneven-query generator at a larger batch, one launch: the few queries that never
     connect bound a 1024-query step, and a larger batch spreads them over more plans."""
     import torch
@@ -98,7 +177,7 @@ def scale_leg(n_queries: int, checker_factory, kw: dict, launch: dict):
     from prrax_amd import prepare_batch
 
     t0 = time.perf_counter()
-    robot, worlds, starts, goals, nodes = build_c3(n_queries, checker_factory)
+    robot, worlds, starts, goals, nodes = build_c3(n_queries, gpu_free_checker(7))
     log(f"scaling: {n_queries} queries built in {time.perf_counter() - t0:.1f}s")
     pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
                        max_nodes_per_query=nodes, **launch, **kw)
@@ -123,11 +202,10 @@ def dual_arm_leg(n_queries: int):
     import torch
 
     from prrax_amd import prepare_batch, scenes
-    from prrax_amd.prrtc import CollisionChecker
 
     sc = scenes.scene_c4()
     cc, robot = sc.context(), sc.robot
-    free = CollisionChecker(cc, robot.dim, collision_checker="differentiable", collision_margin=1e-5).is_free
+    free = gpu_free_checker(robot.dim)(cc)
     starts, goals = [], []
     for qi in range(n_queries):
         rng = np.random.default_rng(scenes.query_seed(4, qi))
@@ -146,20 +224,12 @@ def dual_arm_leg(n_queries: int):
     dt = time.perf_counter() - t
     solved = pb.bufs.host_summary()["solved"] == 1
     singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=cc, **kw) for i in range(n_queries)]
-    singles[0].enqueue()
-    torch.cuda.synchronize()
-    times = []
-    for sp in singles:
-        t = time.perf_counter()
-        sp.enqueue()
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t)
-    solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
+    times, solved_pq = time_singles(singles, "dual arm")
     out = {"workload": f"{n_queries} queries, 14-DOF dual arm (128 spheres, mimic fingers), 150 spheres + 12 boxes "
                        "+ floor, 1-4 goals, step 0.3, S 128, max_nodes 32768",
            "batched_plans_per_s": round(n_queries / dt, 2), "batched_ms": round(dt * 1000, 2),
            "solve_rate": round(float(solved.mean()), 4),
-           "per_query_plans_per_s": round(n_queries / sum(times), 2),
+           "per_query_plans_per_s": round(n_queries / times.sum(), 2),
            "per_query_p50_ms": round(1000 * float(np.median(times)), 3),
            "identical_solved_flags": bool(np.array_equal(solved, solved_pq))}
     log(f"dual arm: {out}")
@@ -169,16 +239,15 @@ def dual_arm_leg(n_queries: int):
 def refine_leg(n_paths: int, waypoints: int = 64, steps: int = 50):
     """Path refinement: n_paths planner paths of the 7-DOF arm in a 100-sphere world, resampled
     to 64 waypoints, 50 gradient steps of the sphere-penetration cost (hand-written FK adjoint),
-    each moved waypoint kept only if the exact predicate passes."""
+    each move kept only if the exact predicate passes at the waypoint and along both edges."""
     import torch
 
     from prrax_amd import prrtc_plan_batch, scenes
-    from prrax_amd.prrtc import CollisionChecker
     from prrax_amd.refine import SphereCost, refine_paths, resample_path
 
     sc = scenes.scene_c5()
     cc, robot = sc.context(), sc.robot
-    free = CollisionChecker(cc, robot.dim, collision_checker="differentiable", collision_margin=1e-5).is_free
+    free = gpu_free_checker(robot.dim)(cc)
     starts, goals = [], []
     for qi in range(n_paths):
         sg = scenes.sample_free(np.random.default_rng(scenes.query_seed(5, qi)), robot, 2, free)
@@ -226,130 +295,26 @@ def refine_leg(n_paths: int, waypoints: int = 64, steps: int = 50):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--queries", type=int, default=1024)
-    ap.add_argument("--samples", type=int, default=128, help="explorers (samples per round) per query")
-    ap.add_argument("--granularity", type=int, default=16)
-    ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--slots", type=int, default=0)
-    ap.add_argument("--per-query", type=int, default=-1, help="queries run as single launches (-1 = all)")
-    ap.add_argument("--fkcc-configs", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=64)
-    ap.add_argument("--max-iterations", type=int, default=10_000)
-    ap.add_argument("--team-max", type=int, default=0, help="largest work-group team per open query (0 = library default)")
-    ap.add_argument("--refine-paths", type=int, default=512, help="paths of the refinement leg (0 = skip)")
-    ap.add_argument("--dual-arm-queries", type=int, default=256, help="queries of the 14-DOF leg (0 = skip)")
-    ap.add_argument("--scale-queries", type=int, default=4096,
-                    help="queries of the batch-size scaling leg (same generator, one launch; 0 = skip)")
-    ap.add_argument("--no-autotune", action="store_true", help="use S = --samples and the library's launch defaults")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: every query is independent and runs on a single device; "
-              "runs on several GPUs are not supported (the query queue would be split per GPU).")
-        sys.exit(2)
-    if args.copy_calibration > 0:
-        copy_calibration(args.copy_calibration)
-        return
-
+def fkcc_leg(n_cfg: int) -> dict:
+    """FK+CC throughput: n_cfg uniform configurations against the large-scene world.  Verdicts
+    (binary; margin 0.005) use the coarse-culled check; the min-distance leg is the flat full
+    sweep over every sphere pair (no early exit, no culling)."""
     import torch
 
-    from prrax_amd import prepare_batch, scenes
+    from prrax_amd import scenes
     from prrax_amd.prrtc import CollisionChecker
 
-    torch.cuda.set_device(0)
-    dev_name = torch.cuda.get_device_name(0)
-    log(f"device {dev_name}")
-
-    def checker_factory(cc):
-        chk = CollisionChecker(cc, 7, collision_checker="differentiable", collision_margin=1e-5)
-        return chk.is_free
-
-    t0 = time.perf_counter()
-    robot, worlds, starts, goals, nodes = build_c3(args.queries, checker_factory)
-    log(f"workload built in {time.perf_counter() - t0:.1f}s")
-    kw = dict(max_iterations=args.max_iterations, step_size=0.5, num_new_samples=args.samples, granularity=args.granularity,
-              min_vals=robot.lower, max_vals=robot.upper)
-    launch = dict(threads=args.threads, slots=args.slots, team_max=args.team_max)
-    tuned = None
-    if not args.no_autotune:
-        from prrax_amd.autotune import autotune
-
-        t0 = time.perf_counter()
-        base = {k: v for k, v in kw.items() if k != "num_new_samples"}
-        tuned = autotune(starts, goals, worlds, max_nodes_per_query=nodes, log=log, **base)
-        kw["num_new_samples"] = tuned.num_new_samples
-        launch.update({k: v for k, v in tuned.launch.items()})
-        log(f"autotune: chose S={tuned.num_new_samples} {tuned.launch} in {time.perf_counter() - t0:.1f}s")
-    # Queue order: the library's default, longest predicted first (results do not depend on it).
-    pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
-                       max_nodes_per_query=nodes, **launch, **kw)
-    # ---- batched
-    for w in range(args.warmup):
-        t = time.perf_counter()
-        pb.enqueue()
-        torch.cuda.synchronize()
-        log(f"warmup step {w}: {time.perf_counter() - t:.3f}s")
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record()
-    for _ in range(args.steps):
-        pb.enqueue()
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = ev0.elapsed_time(ev1) / args.steps
-    ms_per_step = wall * 1000.0 / args.steps
-    summ = pb.bufs.host_summary()
-    stats = pb.bufs.stats()
-    solved_b = summ["solved"] == 1
-    plans_per_s = args.queries / (ms_per_step / 1000.0)
-    log(f"batched: {ms_per_step:.2f} ms/step ({kernel_ms:.2f} ms device), {plans_per_s:.1f} plans/s, "
-        f"solve rate {solved_b.mean():.4f}, {stats}")
-    # ---- per-query launches
-    n_pq = args.queries if args.per_query < 0 else min(args.per_query, args.queries)
-    pq = None
-    if n_pq > 0:
-        singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=[worlds[i]], max_nodes=nodes[i],
-                                 threads=launch["threads"], **kw) for i in range(n_pq)]
-        singles[0].enqueue()
-        torch.cuda.synchronize()
-        times = []
-        for k, sp in enumerate(singles):
-            t = time.perf_counter()
-            sp.enqueue()
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - t)
-            if (k + 1) % 128 == 0:
-                log(f"per-query launches: {k + 1}/{n_pq} done, {sum(times):.2f}s")
-        solved_pq = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
-        pq_plans_per_s = n_pq / sum(times)
-        same = bool(np.array_equal(solved_pq, solved_b[:n_pq]))
-        log(f"per-query: {pq_plans_per_s:.1f} plans/s, p50 {1000 * np.median(times):.3f} ms, solve rate "
-            f"{solved_pq.mean():.4f}, identical solved flags: {same}")
-        pq = {"plans_per_s": round(pq_plans_per_s, 2), "p50_ms_per_plan": round(1000 * float(np.median(times)), 3),
-              "solve_rate": round(float(solved_pq.mean()), 4), "queries": n_pq, "identical_solved_flags": same}
-    # ---- FK + CC throughput (large-scene world, 10^6 configs)
     sc2 = scenes.scene_c2()
     rng = np.random.default_rng(11)
-    qcfg = torch.from_numpy(rng.uniform(sc2.robot.lower, sc2.robot.upper,
-                                        size=(args.fkcc_configs, 7)).astype(np.float32)).cuda()
+    qcfg = torch.from_numpy(rng.uniform(sc2.robot.lower, sc2.robot.upper, size=(n_cfg, 7)).astype(np.float32)).cuda()
     fkcc = {}
-    # verdicts (binary; margin 0.005) use the coarse-culled check; the min-distance leg is the
-    # flat full sweep over every sphere pair (no early exit, no culling)
     legs = (("binary", "binary", True), ("margin_verdict", "differentiable", True),
             ("min_distance_full_sweep", "differentiable", False))
-    for label, mode, verdict in (legs if args.fkcc_configs > 0 else ()):
+    for label, mode, verdict in legs:
         # margin 0.005: the floor lies 0.01 m under the base spheres, so 0.02 flags every config
         chk = CollisionChecker(sc2.context(), 7, collision_checker=mode, collision_margin=0.005)
-        hit = torch.empty(args.fkcc_configs, dtype=torch.uint8, device="cuda") if verdict else None
-        mind = None if verdict else torch.empty(args.fkcc_configs, dtype=torch.float32, device="cuda")
+        hit = torch.empty(n_cfg, dtype=torch.uint8, device="cuda") if verdict else None
+        mind = None if verdict else torch.empty(n_cfg, dtype=torch.float32, device="cuda")
         flops = torch.zeros(1, dtype=torch.int64, device="cuda")
         chk.enqueue(qcfg, hit, mind)
         torch.cuda.synchronize()
@@ -363,18 +328,228 @@ def main():
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         fl = float(flops.item()) / reps
-        fkcc[label] = {"cfg_per_s": round(args.fkcc_configs / (ms / 1000.0), 1), "ms": round(ms, 3),
+        fkcc[label] = {"cfg_per_s": round(n_cfg / (ms / 1000.0), 1), "ms": round(ms, 3),
                        "tflops": round(fl / (ms / 1000.0) / 1e12, 3),
                        "frac_of_fp32_peak": round(fl / (ms / 1000.0) / 1e12 / PEAK_FP32_TFLOPS, 4)}
         if verdict:
             fkcc[label]["in_collision"] = round(float(hit.float().mean().item()), 4)
         log(f"fkcc {label}: {fkcc[label]}")
-    scale = scale_leg(args.scale_queries, checker_factory, kw, launch) if args.scale_queries > 0 else None
+    return {"workload": f"{n_cfg} uniform configs vs large-scene world (64 spheres, 200 self pairs, 200 spheres, "
+                        "16 boxes, 8 capsules, floor)", **fkcc}
+
+
+# ------------------------------------------------------------------------------ CPU baselines
+def _one_thread():
+    from threadpoolctl import threadpool_limits
+
+    threadpool_limits(1)
+
+
+def _host_plan_job(job):
+    """One query of the reference's host RRT-Connect loop (NumPy restatement); runs in a worker
+    process forked before the GPU was touched."""
+    from oracle import host_rrtc
+
+    cc, start, goals, kw, limit = job
+    t = time.perf_counter()
+    r = host_rrtc.plan(host_rrtc.BatchChecker(cc, len(start)), start, goals, time_limit=limit, **kw)
+    return {"solved": bool(r["solved"]), "timed_out": bool(r["timed_out"]), "s": time.perf_counter() - t}
+
+
+def cpu_host_loop_leg(pool, workers: int, worlds, starts, goals, nodes, kw, sample, limit: float) -> dict:
+    jobs = [(worlds[i], starts[i], goals[i],
+             dict(max_iterations=kw["max_iterations"], step_size=kw["step_size"], num_new_samples=128,
+                  granularity=kw["granularity"], max_nodes=nodes[i], min_vals=kw["min_vals"], max_vals=kw["max_vals"],
+                  seed=int(i)), limit) for i in sample]
+    t0 = time.perf_counter()
+    res = pool.map(_host_plan_job, jobs, chunksize=1)
+    dt = time.perf_counter() - t0
+    solved = [r["solved"] for r in res]
+    return {"value": round(len(jobs) / dt, 3), "unit": "plans/s", "cores": workers, "kind": "port",
+            "sample": f"{len(jobs)} of the {len(worlds)} queries (the queries still open after the probe's 24 rounds, "
+                      f"then every {max(1, len(worlds) // max(1, len(jobs)))}th), the reference's host "
+                      "RRT-Connect loop (prrtc_robogpu.py:183-264) with batched float32 NumPy FK + collision, "
+                      f"S 128, G 16, one query per worker process, {limit:.0f} s wall limit per query",
+            "solve_rate": round(float(np.mean(solved)), 4),
+            "time_limited": int(sum(r["timed_out"] for r in res)),
+            "p50_s_per_plan": round(float(np.median([r["s"] for r in res])), 3), "wall_s": round(dt, 2)}
+
+
+def cpu_single_query_leg(pool, n: int) -> dict:
+    """The plain single query on the CPU path (7-DOF arm, 48 spheres, 60 self pairs, 40 world
+    spheres + floor, S 128, 16384 nodes): n seeded queries, one process each, p50 ms per plan."""
+    from prrax_amd import scenes
+    from prrax_amd.validate import min_distance
+
+    sc = scenes.scene_c1()
+    cc = sc.context()
+    jobs = []
+    for qi in range(n):
+        rng = np.random.default_rng(scenes.query_seed(1, qi))
+        sg = scenes.sample_free(rng, sc.robot, 2, lambda q: min_distance(cc, np.asarray(q)[None])[0] > 1e-5)
+        jobs.append((cc, sg[0], sg[1:], dict(max_iterations=10_000, step_size=0.5, num_new_samples=128, granularity=16,
+                                             max_nodes=16384, min_vals=sc.robot.lower, max_vals=sc.robot.upper,
+                                             seed=qi), 60.0))
+    res = pool.map(_host_plan_job, jobs, chunksize=1)
+    s = [r["s"] for r in res]
+    return {"workload": f"{n} single queries, 7-DOF arm (48 spheres, 60 self pairs), 40 world spheres + floor, S 128, "
+                        "max_nodes 16384, one thread each (host RRT-Connect loop, NumPy)",
+            "p50_ms_per_plan": round(1000 * float(np.median(s)), 1), "solve_rate": round(float(np.mean(
+                [r["solved"] for r in res])), 4)}
+
+
+def _lockstep_job(args):
+    cc, start, goals, kw = args
+    from oracle import oracle as O
+
+    r = O.plan(O.Context(cc, 7), start, goals, **kw)
+    return r["status"] == O.STATUS_SOLVED
+
+
+def cpu_lockstep_leg(worlds, starts, goals, nodes, kw, sample, workers: int) -> dict:
+    """The C restatement of the kernel's lockstep algorithm, one query per thread (ctypes releases
+    the GIL for each oracle call)."""
+    jobs = [(worlds[i], starts[i], goals[i], dict(kw, max_nodes=nodes[i])) for i in sample]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        solved = list(ex.map(_lockstep_job, jobs))
+    dt = time.perf_counter() - t0
+    return {"value": round(len(jobs) / dt, 3), "unit": "plans/s", "cores": workers,
+            "sample": f"{len(jobs)} evenly spaced queries (not the probe's tail), the C restatement of the "
+                      "kernels' lockstep algorithm",
+            "solve_rate": round(float(np.mean(solved)), 4), "wall_s": round(dt, 2)}
+
+
+def copy_calibration(mib: int):
+    """One plain copy of ``mib`` MiB: its FETCH_SIZE / WRITE_SIZE readings calibrate the HBM counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {mib} MiB read + {mib} MiB written")
+
+
+# ------------------------------------------------------------------------------ main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--samples", type=int, default=128, help="explorers per round when not autotuned")
+    ap.add_argument("--granularity", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--team-max", type=int, default=0)
+    ap.add_argument("--fk-batch", type=int, default=0)
+    ap.add_argument("--per-query", type=int, default=-1, help="queries run as single launches (-1 = all)")
+    ap.add_argument("--fkcc-configs", type=int, default=1_000_000)
+    ap.add_argument("--max-iterations", type=int, default=10_000)
+    ap.add_argument("--c2-queries", type=int, default=100, help="queries of the large-scene leg (0 = skip)")
+    ap.add_argument("--refine-paths", type=int, default=512, help="paths of the refinement leg (0 = skip)")
+    ap.add_argument("--dual-arm-queries", type=int, default=256, help="queries of the 14-DOF leg (0 = skip)")
+    ap.add_argument("--scale-queries", type=int, default=4096,
+                    help="queries of the batch-size scaling leg (same generator, one launch; 0 = skip)")
+    ap.add_argument("--no-validate", action="store_true", help="skip the FP64 re-validation of the paths")
+    ap.add_argument("--no-autotune", action="store_true", help="use S = --samples and the library's launch defaults")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="queries of the host-loop CPU baseline")
+    ap.add_argument("--cpu-limit", type=float, default=20.0, help="wall-time limit per CPU-baseline query (s)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every query is independent and runs on a single device; "
+              "runs on several GPUs are not supported (the query queue would be split per GPU).")
+        sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
+
+    # The CPU-baseline workers are forked now, before anything touches the GPU; each runs one query
+    # at a time on one thread (their BLAS pools are limited to a thread each).
+    workers = cpu_workers()
+    pool = None if args.no_cpu else multiprocessing.get_context("fork").Pool(workers, initializer=_one_thread)
+
+    import torch
+
+    from prrax_amd import prepare_batch
+
+    torch.cuda.set_device(0)
+    dev_name = torch.cuda.get_device_name(0)
+    log(f"device {dev_name}")
+    t0 = time.perf_counter()
+    robot, worlds, starts, goals, nodes = build_c3(args.queries, gpu_free_checker(7))
+    log(f"workload built in {time.perf_counter() - t0:.1f}s")
+    kw = dict(max_iterations=args.max_iterations, step_size=0.5, num_new_samples=args.samples,
+              granularity=args.granularity, min_vals=robot.lower, max_vals=robot.upper)
+    launch = dict(threads=args.threads, slots=args.slots, team_max=args.team_max, fk_batch=args.fk_batch)
+    tuned = None
+    if not args.no_autotune:
+        from prrax_amd.autotune import autotune
+
+        t0 = time.perf_counter()
+        base = {k: v for k, v in kw.items() if k != "num_new_samples"}
+        tuned = autotune(starts, goals, worlds, max_nodes_per_query=nodes, log=log, **base)
+        kw["num_new_samples"] = tuned.num_new_samples
+        launch.update(tuned.launch)
+        log(f"autotune: chose S={tuned.num_new_samples} {tuned.launch} in {time.perf_counter() - t0:.1f}s")
+    pb = prepare_batch(np.stack(starts), goals, collision_context=worlds, max_nodes=max(nodes),
+                       max_nodes_per_query=nodes, **launch, **kw)
+    # ---- batched
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        pb.enqueue()
+        torch.cuda.synchronize()
+        log(f"warmup step {w}: {time.perf_counter() - t:.3f}s")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        pb.enqueue()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1000.0 / args.steps
+    summ = pb.bufs.host_summary()
+    stats = pb.bufs.stats()
+    results = pb.bufs.results()
+    solved_b = summ["solved"] == 1
+    plans_per_s = args.queries / (ms_per_step / 1000.0)
+    log(f"batched: {ms_per_step:.2f} ms/step ({kernel_ms:.2f} ms device), {plans_per_s:.1f} plans/s, "
+        f"solve rate {solved_b.mean():.4f}, {stats}")
+    validation = None if args.no_validate else validate_leg(results, worlds)
+    # ---- per-query launches
+    n_pq = args.queries if args.per_query < 0 else min(args.per_query, args.queries)
+    pq = None
+    if n_pq > 0:
+        pq_launch = {k: v for k, v in launch.items() if k in ("threads", "fk_batch")}
+        singles = [prepare_batch(starts[i][None], [goals[i]], collision_context=[worlds[i]], max_nodes=nodes[i],
+                                 **pq_launch, **kw) for i in range(n_pq)]
+        times, solved_pq = time_singles(singles, "per-query")
+        same = bool(np.array_equal(solved_pq, solved_b[:n_pq]))
+        pq = {"plans_per_s": round(n_pq / times.sum(), 2), "p50_ms_per_plan": round(1000 * float(np.median(times)), 3),
+              "solve_rate": round(float(solved_pq.mean()), 4), "queries": n_pq, "identical_solved_flags": same,
+              # the batched step cannot end before its longest query, which the per-query launch
+              # already runs with the whole GPU: batched / per-query <= sum / max of these times
+              "ratio_bound_sum_over_max": round(float(times.sum() / times.max()), 2),
+              "longest_ms": round(1000 * float(times.max()), 1)}
+        log(f"per-query: {pq}")
+        del singles
+    fkcc = fkcc_leg(args.fkcc_configs) if args.fkcc_configs > 0 else None
+    c2 = c2_leg(args.c2_queries, {k: v for k, v in kw.items() if k in ("max_iterations", "step_size", "granularity")}) \
+        if args.c2_queries > 0 else None
+    scale = scale_leg(args.scale_queries, kw, launch) if args.scale_queries > 0 else None
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
-    default_workload = (args.queries == 1024 and kw["num_new_samples"] == 128 and args.granularity == 16
-                        and args.max_iterations == 10_000)
+    nn_gbs = stats["nn_bytes"] / (kernel_ms / 1000.0) / 1e9
     out = {
         "metric": METRIC,
         "value": round(plans_per_s, 2),
@@ -392,32 +567,51 @@ def main():
                                "max_nodes 8192-65536), one persistent launch",
                    "queries": args.queries, "num_new_samples": kw["num_new_samples"], "granularity": args.granularity,
                    "step_size": 0.5, "max_iterations": args.max_iterations, "launch": stats.get("launch"),
-                   "autotuned": tuned is not None},
+                   "fk_batch": launch.get("fk_batch", 0), "autotuned": tuned is not None},
         "batched": {"plans_per_s": round(plans_per_s, 2), "solve_rate": round(float(solved_b.mean()), 4),
                     "p50_ms_per_plan_device": round(float(np.median(summ["kernel_time_ms"])), 3),
                     "kernel_ms_per_step": round(kernel_ms, 3), "configs_checked": stats["configs"],
-                    "rounds": stats["rounds"]},
+                    "rounds": stats["rounds"], "nn_bytes_per_step": stats["nn_bytes"],
+                    "nn_tile_passes": stats["nn_tile_passes"]},
+        "validation": validation,
         "per_query": pq,
         "batched_over_per_query": round(plans_per_s / pq["plans_per_s"], 2) if pq else None,
-        "fkcc": {"workload": f"{args.fkcc_configs} uniform configs vs large-scene world (64 spheres, 200 self pairs, "
-                             "200 spheres, 16 boxes, 8 capsules, floor)", **fkcc},
+        "fkcc": fkcc,
+        "large_scene": c2,
         "autotune": None if tuned is None else {
             "chosen": tuned.kwargs(), "sample_plans_per_s": round(tuned.plans_per_s, 1),
+            "sample_queries": len(tuned.sample), "tail_after_probe": len(tuned.tail),
+            "tail_in_sample": len(set(tuned.sample) & set(tuned.tail)),
             "candidates": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()} for r in tuned.table]},
         "batch_scaling": scale,
         "refine": refine,
         "dual_arm": dual,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-                     "traffic": TRAFFIC_BYTES_PER_STEP if default_workload else None,
-                     "flops_per_launch": stats["flops"]},
+                     "traffic": TRAFFIC["bytes_per_step"], "traffic_source": TRAFFIC["source"],
+                     "flops_per_launch": stats["flops"], "nn_bytes_per_launch": stats["nn_bytes"],
+                     "nn_achieved_gbs": round(nn_gbs, 2), "nn_frac_of_hbm": round(nn_gbs / PEAK_HBM_GBS, 5)},
         "device": dev_name,
     }
-    if not args.no_cpu:
-        workers = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {args.cpu_sample} queries on {workers} threads")
-        out["cpu_baseline"] = cpu_baseline(worlds, starts, goals, nodes, kw, min(args.cpu_sample, args.queries),
-                                           workers)
+    if pool is not None:
+        info = host_info()
+        order = list(tuned.tail) if tuned is not None else []
+        stride = max(1, args.queries // max(1, args.cpu_sample - min(8, len(order))))
+        sample = list(dict.fromkeys(order[:8] + list(range(0, args.queries, stride))))[:args.cpu_sample]
+        log(f"cpu baseline: host loop on {len(sample)} queries, {workers} processes ({info})")
+        cpu = cpu_host_loop_leg(pool, workers, worlds, starts, goals, nodes, kw, sample, args.cpu_limit)
+        cpu.update(info)
+        log(f"cpu baseline: {cpu}")
+        out["cpu_baseline"] = cpu
+        out["cpu_single_query"] = cpu_single_query_leg(pool, 8)
+        log(f"cpu single query: {out['cpu_single_query']}")
+        # the lockstep oracle runs the kernels' own algorithm; the tail queries (trees filled over
+        # thousands of rounds) would take minutes each, so it times the evenly spaced part only
+        even = [i for i in sample if i not in set(order)]
+        out["cpu_lockstep_oracle"] = cpu_lockstep_leg(worlds, starts, goals, nodes, kw, even, workers)
+        log(f"cpu lockstep oracle: {out['cpu_lockstep_oracle']}")
+        pool.close()
+        pool.join()
     print(json.dumps(out), flush=True)
 
 

@@ -8,9 +8,9 @@
  *
  * Each entry point replaces one XLA FFI handler of the reference; the mapping is given on the
  * declaration.  The reference's argument groups are kept, in order, as structs:
- *   26 buffer arguments  -> start/goals/min/max + prrtc_robot + prrtc_world (+ coarse arrays,
- *                           accepted and ignored: the coarse checkers return the flat binary
- *                           verdict, prrtc_planner.cu:421-425)
+ *   26 buffer arguments  -> start/goals/min/max + prrtc_robot + prrtc_world (world primitives,
+ *                           self pairs and the six coarse arrays; batch calls add the reference's
+ *                           five per-query count vectors)
  *   10 results           -> prrtc_outputs (first ten fields, same order and dtypes)
  *   14 attributes        -> prrtc_attrs (same order and dtypes)
  */
@@ -60,11 +60,15 @@ typedef struct {
     int32_t n_robot_spheres;
 } prrtc_robot;
 
-/* World primitives + self pairs: arguments 16-20 of PrrtcPlannerFfi (prrtc_planner.cu:1086-1090).
- * Single-query calls: arrays hold n_* rows.  Batch calls with per-query worlds
- * (PrrtcPlannerBatchCtxFfi, prrtc_planner.cu:1946-2005): arrays are padded [batch, n_*, F] and
- * counts is [batch, 5] (spheres, capsules, boxes, halfspaces, self pairs) -- the reference passes
- * these as five [batch] vectors; here they are interleaved per query. */
+/* World primitives, self pairs and the coarse-checker arrays: arguments 16-26 of PrrtcPlannerFfi
+ * (prrtc_planner.cu:1086-1096), in the same order.  Single-query calls: arrays hold n_* rows.
+ * Batch calls with per-query worlds (PrrtcPlannerBatchCtxFfi, prrtc_planner.cu:1946-1980): arrays
+ * are padded [batch, n_*, F] (n_* = the padded capacity) and the five per-query counts are the
+ * reference's five device i32 [batch] vectors (world_spheres_count ... self_pairs_count, arguments
+ * 21-25 there); a NULL count vector means every query uses n_*.
+ * The six coarse arrays (utils.py:160-227; prrtc.py:76-110) are accepted and checked for
+ * consistent sizes; they change no result: the culled check derives its own per-frame bounding
+ * spheres (collision modes 2/3 return the binary verdict, prrtc_planner.cu:421-425). */
 typedef struct {
     const float* world_spheres;    /* [.., 4]  (x, y, z, r) */
     const float* world_capsules;   /* [.., 7]  (p1, p2, r) */
@@ -72,7 +76,18 @@ typedef struct {
     const float* world_halfspaces; /* [.., 6]  (n, p); free side is +n */
     const int32_t* self_pairs;     /* [.., 2] */
     int32_t n_world_spheres, n_world_capsules, n_world_boxes, n_world_halfspaces, n_self_pairs;
-    const int32_t* counts;         /* batch only: [batch, 5] or NULL (every query uses n_*) */
+    const int32_t* world_spheres_count;    /* batch only: [batch] or NULL */
+    const int32_t* world_capsules_count;   /* batch only: [batch] or NULL */
+    const int32_t* world_boxes_count;      /* batch only: [batch] or NULL */
+    const int32_t* world_halfspaces_count; /* batch only: [batch] or NULL */
+    const int32_t* self_pairs_count;       /* batch only: [batch] or NULL */
+    const int32_t* coarse_sphere_link_idx; /* [n_coarse_spheres] */
+    const float* coarse_sphere_local;      /* [n_coarse_spheres, 3] */
+    const float* coarse_sphere_radius;     /* [n_coarse_spheres] */
+    const int32_t* fine_link_offsets;      /* [n_fine_link_offsets] */
+    const int32_t* coarse_self_pairs;      /* [n_coarse_self_pairs, 2] */
+    const int32_t* self_pair_ranges;       /* [n_self_pair_ranges, 2] */
+    int32_t n_coarse_spheres, n_fine_link_offsets, n_coarse_self_pairs, n_self_pair_ranges;
 } prrtc_world;
 
 /* The 14 FFI attributes, same order (prrtc_planner.cu:2103-2116, prrtc.py:537-550). */
@@ -123,18 +138,22 @@ typedef struct {
     int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (512) */
     int32_t epoch_rounds; /* planner: most rounds a query runs between two re-formings of the teams of
                              the team pass (epochs run 4, 8, 16, ... rounds up to this); 0 = auto (64) */
+    int32_t fk_batch;     /* planner: configurations per wave in one FK + check tile (8 or 16; a check
+                             work unit covers at most this many waypoints); 0 = auto (16, or 8 when
+                             that lets more work-groups stay resident) */
 } prrtc_launch;
 
 /* Workspace bytes a planner call needs (radii + per-query explorer state + queue counters). */
 size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
 
-/* Byte offset, inside that workspace, of 16 uint64 counters the last planner call left there:
+/* Byte offset, inside that workspace, of 32 uint64 counters the last planner call left there:
  * [0] configurations FK+collision-checked, [1] lockstep rounds summed over queries,
  * [2] algorithmic FP32 flops of that checking (FMA = 2), [3] launch shape chosen (bits 0-15
  * resident work-groups, 16-31 threads per work-group, 32-47 pass-1 rounds, 48-63 team cap),
  * [4..9] / [10..15] device wall-clock ticks spent in the round phases of the per-query pass /
- * the team pass (sample+NN, extend check with the connect NN, commit, unused, connect walk,
- * connect commit).
+ * the team pass (sample + NN, extend checks, commit, connect NN, connect walk, connect commit),
+ * [16] algorithmic bytes of the nearest-neighbour scans (4 * dim per node per pass of a tile of
+ * up to 16 queries), [17] such tile passes, [18..31] zero.
  * No reference counterpart (the reference reports only kernel_time_ms). */
 size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
 
@@ -162,8 +181,9 @@ int prrtc_planner_noalloc(prrtc_stream_t stream, const float* start, const float
                           const prrtc_launch* launch);
 
 /* Batch of independent queries in one persistent launch.  Replaces PrrtcPlannerBatchFfi
- * (prrtc_planner.cu:2007-2061; shared world: world->counts == NULL and world_per_query == 0) and
- * PrrtcPlannerBatchCtxFfi (prrtc_planner.cu:1946-2005; padded per-query worlds, world_per_query=1).
+ * (prrtc_planner.cu:2007-2061; one shared world: world_per_query == 0, count vectors ignored) and
+ * PrrtcPlannerBatchCtxFfi (prrtc_planner.cu:1946-2005; padded per-query worlds with the five
+ * count vectors: world_per_query == 1).
  * starts [batch, dim], goals [batch, max_goals, dim].  Extensions over the reference (all may be
  * NULL): goal_counts [batch] (uneven goal sets), max_nodes_q [batch] (per-query tree capacity,
  * <= attrs->max_nodes), seeds [batch] (sampler seed; 0 = the reference's sequence), order [batch]

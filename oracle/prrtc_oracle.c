@@ -256,7 +256,7 @@ void oc_fk(const oc_context* c, const float* q, float* T_out) {
 /* ---------------------------------------------------------------- collision ------------ */
 /* Binary tests: the reference's "dist <= 0" (prrtc_planner.cu:250-304) evaluated without square
  * roots.  A sphere or capsule whose radius sum rr = r + R is >= 0 is hit iff d^2 <= rr^2; with
- * rr < 0 never (sqrt(d^2) - rr > 0).  A box with sr >= 0 is hit iff the squared outside distance
+ * rr < 0 never (sqrt(d^2) - rr > 0): both at once as d^2 <= rr |rr|.  A box with sr >= 0 is hit iff the squared outside distance
  * is <= sr^2 (it is 0 when the centre is inside); with sr < 0 iff the centre lies deeper than
  * |sr| inside (max_i q_i <= sr, q_i = |l_i| - h_i).  A halfspace iff n.(c - p) <= sr.  These
  * agree with the signed-distance form except within rounding of contact. */
@@ -267,7 +267,7 @@ static int hit_sphere(float sx, float sy, float sz, float sr, const float* o) {
 #if OC_PLANT == 11
     return d2 <= rr * rr;
 #else
-    return rr >= 0.0f && d2 <= rr * rr;
+    return d2 <= rr * fabsf(rr);
 #endif
 }
 
@@ -284,7 +284,7 @@ static float capsule_d2(float sx, float sy, float sz, const float* cap) {
 
 static int hit_capsule(float sx, float sy, float sz, float sr, const float* cap) {
     const float rr = sr + cap[6];
-    return rr >= 0.0f && capsule_d2(sx, sy, sz, cap) <= rr * rr;
+    return capsule_d2(sx, sy, sz, cap) <= rr * fabsf(rr);
 }
 
 /* q_i = |l_i| - h_i of the centre in the box frame; returns the squared outside distance. */
@@ -313,18 +313,18 @@ static int hit_halfspace(float sx, float sy, float sz, float sr, const float* h)
 
 /* Margin tests: the reference's "dist < margin" (prrtc_planner.cu:203-206, 314-387), also
  * without square roots.  With L = r + R + margin, a sphere or capsule is near iff L > 0 and
- * d^2 < L^2; a box (L = sr + margin) iff L > 0 and outside^2 < L^2 when the centre is outside
+ * d^2 < L^2 (d^2 < L |L|); a box (L = sr + margin) iff L > 0 and outside^2 < L^2 when the centre is outside
  * (outside^2 > 0), else iff min(max_i q_i, 0) < L; a halfspace iff n.(c - p) - sr < margin. */
 static int near_sphere(float sx, float sy, float sz, float sr, const float* o, float margin) {
     const float dx = sx - o[0], dy = sy - o[1], dz = sz - o[2];
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float L = (sr + o[3]) + margin;
-    return L > 0.0f && d2 < L * L;
+    return d2 < L * fabsf(L);
 }
 
 static int near_capsule(float sx, float sy, float sz, float sr, const float* cap, float margin) {
     const float L = (sr + cap[6]) + margin;
-    return L > 0.0f && capsule_d2(sx, sy, sz, cap) < L * L;
+    return capsule_d2(sx, sy, sz, cap) < L * fabsf(L);
 }
 
 static int near_box(float sx, float sy, float sz, float sr, const float* b, float margin) {

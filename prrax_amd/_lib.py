@@ -64,7 +64,12 @@ class World(ctypes.Structure):
         ("n_world_spheres", ctypes.c_int32), ("n_world_capsules", ctypes.c_int32),
         ("n_world_boxes", ctypes.c_int32), ("n_world_halfspaces", ctypes.c_int32),
         ("n_self_pairs", ctypes.c_int32),
-        ("counts", _vp),
+        ("world_spheres_count", _vp), ("world_capsules_count", _vp), ("world_boxes_count", _vp),
+        ("world_halfspaces_count", _vp), ("self_pairs_count", _vp),
+        ("coarse_sphere_link_idx", _vp), ("coarse_sphere_local", _vp), ("coarse_sphere_radius", _vp),
+        ("fine_link_offsets", _vp), ("coarse_self_pairs", _vp), ("self_pair_ranges", _vp),
+        ("n_coarse_spheres", ctypes.c_int32), ("n_fine_link_offsets", ctypes.c_int32),
+        ("n_coarse_self_pairs", ctypes.c_int32), ("n_self_pair_ranges", ctypes.c_int32),
     ]
 
 
@@ -89,7 +94,7 @@ class Outputs(ctypes.Structure):
 
 class Launch(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int32), ("slots", ctypes.c_int32), ("pass1_rounds", ctypes.c_int32),
-                ("team_max", ctypes.c_int32), ("epoch_rounds", ctypes.c_int32)]
+                ("team_max", ctypes.c_int32), ("epoch_rounds", ctypes.c_int32), ("fk_batch", ctypes.c_int32)]
 
 
 class PrrtcError(RuntimeError):

@@ -1,19 +1,26 @@
-"""Autotuner for batched planning: samples per round and launch shape.
+"""Autotuner for batched planning: samples per round, waves per query, and the FK/check tile.
 
-The launch shape (threads per work-group = waves per work-group, team cap, pass-1 budget) never
-changes a plan.  The number of samples per round (``num_new_samples``, S) does: it is the
-reference's own parameter (prrtc.py:258), and a different S gives different (equally valid)
-trees, so a candidate S is only accepted if its solve rate on the tuning sample is at least the
-reference default's (S = 128).
+Knobs (every one but S leaves the plans unchanged):
 
-Tuning runs the real planner on a stratified subset of the batch: every k-th query plus the
-queries with the most obstacles (the long ones), and keeps the candidate with the highest
-plans/s.
+* ``num_new_samples`` (S, the reference's own parameter, prrtc.py:258): explorers per round.  A
+  different S gives different (equally valid) trees, so a candidate S is accepted only if its
+  solve rate on the tuning sample is at least the reference default's (S = 128).
+* ``threads`` (waves per work-group) and ``team_max`` (the largest team of work-groups one open
+  query gets): together the waves per query.
+* ``fk_batch`` (8 or 16 configurations per wave): the tile of one FK + collision-check step;
+  a check work unit covers at most this many waypoints.
+
+The makespan of an uneven batch is set by its few longest queries (those that never connect
+run until their trees are full), so the tuning sample must hold them: a short probe launch of
+the whole batch (``probe_rounds`` rounds) finds the queries still open after it, and the sample
+is those plus an evenly spaced subset of the rest, as many queries as the GPU holds work-groups
+(256), so that the one-work-group pass is loaded as in the whole batch.  Each candidate then
+plans the sample with the real settings, and the fastest at an accepted solve rate wins.  The search is staged: S
+first at the library's launch defaults, then the launch shape at the chosen S.
 """
 
 from __future__ import annotations
 
-import itertools
 import time
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
@@ -33,51 +40,79 @@ class TuneResult:
     plans_per_s: float
     solve_rate: float
     table: list = field(default_factory=list)
+    sample: list = field(default_factory=list)
+    tail: list = field(default_factory=list)
 
     def kwargs(self) -> dict:
         """Keyword arguments for ``prepare_batch`` / ``prrtc_plan_batch``."""
         return dict(num_new_samples=self.num_new_samples, **self.launch)
 
 
-def tuning_sample(contexts: Sequence[dict], n: int = 96, n_heavy: int = 16) -> list[int]:
-    """Indices of a stratified subset: ``n_heavy`` queries with the most world primitives plus
-    evenly spaced others, ``n`` in total."""
-    B = len(contexts)
-    if B <= n:
-        return list(range(B))
-    load = np.array([c["world_spheres"].shape[0] + 4 * c["world_boxes"].shape[0] + 2 * c["world_capsules"].shape[0]
-                     for c in contexts])
-    heavy = list(np.argsort(-load, kind="stable")[:n_heavy])
-    rest = [i for i in np.linspace(0, B - 1, n - n_heavy).astype(int) if i not in heavy]
-    return sorted(set(int(i) for i in heavy + rest))[:n]
-
-
-def autotune(starts, goals, collision_context, *, max_nodes_per_query=None, samples: Sequence[int] = (128,),
-             threads: Sequence[int] = (256, 512), pass1_rounds: Sequence[int] = (24,),
-             team_max: Sequence[int] = (0,), n_sample: int = 96, repeats: int = 1,
-             log=None, **plan_kw) -> TuneResult:
-    """Times every candidate on a tuning sample of the batch and returns the fastest whose solve
-    rate is at least the default S's.  ``plan_kw`` are the planner keywords of the real run
-    (max_iterations, step_size, limits, ...); ``collision_context`` is one dict or one per query.
-
-    By default only the work-group size is tuned (S stays 128, the team cap stays the library's):
-    a sample rarely holds the few queries that never connect, and those set a batch's makespan,
-    so an S or a team cap chosen on the sample can lose on the whole batch.  ``pass1_rounds`` is pinned (24, the full-batch default) so the
-    sample runs in the same regime as a batch larger than the resident work-groups."""
-    B = len(starts)
-    ctxs = collision_context if isinstance(collision_context, (list, tuple)) else [collision_context] * B
-    idx = tuning_sample(ctxs, n_sample)
+def _subset(starts, goals, ctxs, shared_ctx, mnq, idx):
     st = np.stack([np.asarray(starts[i], np.float32) for i in idx])
     gl = [goals[i] for i in idx]
-    cx = [ctxs[i] for i in idx] if isinstance(collision_context, (list, tuple)) else collision_context
-    mn = None
+    cx = shared_ctx if shared_ctx is not None else [ctxs[i] for i in idx]
+    mn = None if mnq is None else [int(mnq[i]) for i in idx]
+    return st, gl, cx, mn
+
+
+def probe_tail(starts, goals, collision_context, *, max_nodes_per_query=None, probe_rounds: int = 24,
+               num_new_samples: int = DEFAULT_SAMPLES, **plan_kw) -> np.ndarray:
+    """Indices of the queries still open after ``probe_rounds`` rounds of the whole batch (one
+    launch), longest trees first."""
+    kw = dict(plan_kw, max_iterations=probe_rounds)
     if max_nodes_per_query is not None:
-        mn = [int(max_nodes_per_query[i]) for i in idx]
-        plan_kw = dict(plan_kw, max_nodes=max(mn))
-    table = []
-    for S, nt, p1, tm in itertools.product(samples, threads, pass1_rounds, team_max):
+        kw["max_nodes"] = int(max(max_nodes_per_query))
+    pb = prepare_batch(np.stack([np.asarray(s, np.float32) for s in starts]), goals,
+                       collision_context=collision_context, max_nodes_per_query=max_nodes_per_query,
+                       num_new_samples=num_new_samples, **kw)
+    pb.enqueue()
+    torch.cuda.synchronize()
+    h = pb.bufs.host_summary()
+    open_q = np.nonzero(h["solved"] != 1)[0]
+    size = h["sizes"].sum(axis=1)
+    return open_q[np.argsort(-size[open_q], kind="stable")]
+
+
+def tuning_sample(n_batch: int, tail: Sequence[int], n: int = 256, n_tail: int = 16) -> list[int]:
+    """The first ``n_tail`` tail queries plus evenly spaced others, ``n`` in total."""
+    if n_batch <= n:
+        return list(range(n_batch))
+    head = [int(i) for i in list(tail)[:n_tail]]
+    rest = [int(i) for i in np.linspace(0, n_batch - 1, n - len(head)).astype(int) if int(i) not in head]
+    return sorted(set(head + rest))[:n]
+
+
+def autotune(starts, goals, collision_context, *, max_nodes_per_query=None,
+             samples: Sequence[int] = (128, 256, 512), threads: Sequence[int] = (256, 512),
+             team_max: Sequence[int] = (0, 128), fk_batch: Sequence[int] = (16, 8), n_sample: int = 256,
+             n_tail: int = 16, probe_rounds: int = 24, repeats: int = 1, log=None, **plan_kw) -> TuneResult:
+    """Times candidates on a tail-seeded sample of the batch and returns the fastest whose solve
+    rate is at least S = 128's.  ``plan_kw`` are the planner keywords of the real run
+    (max_iterations, step_size, limits, ...); ``collision_context`` is one dict or one per query."""
+    B = len(starts)
+    shared = None if isinstance(collision_context, (list, tuple)) else collision_context
+    ctxs = collision_context if shared is None else None
+    tail = probe_tail(starts, goals, collision_context, max_nodes_per_query=max_nodes_per_query,
+                      probe_rounds=probe_rounds, **plan_kw)
+    idx = tuning_sample(B, tail, n_sample, n_tail)
+    st, gl, cx, mn = _subset(starts, goals, ctxs, shared, max_nodes_per_query, idx)
+    kw = dict(plan_kw)
+    if mn is not None:
+        kw["max_nodes"] = max(mn)
+    if log:
+        log(f"autotune: {len(tail)} queries open after {probe_rounds} rounds; sample of {len(idx)} holds "
+            f"{sum(1 for i in idx if i in set(int(t) for t in tail))} of them")
+    table: list[dict] = []
+
+    def run(S, nt, tm, fb) -> dict:
+        for r in table:
+            if (r["num_new_samples"], r["threads"], r["team_max"], r["fk_batch"]) == (S, nt, tm, fb):
+                return r
+        # pass 1 runs 24 rounds, as for a batch that fills the resident work-groups (a sample
+        # smaller than that would otherwise start every query on a team)
         pb = prepare_batch(st, gl, collision_context=cx, max_nodes_per_query=mn, num_new_samples=S, threads=nt,
-                           pass1_rounds=p1, team_max=tm, **plan_kw)
+                           team_max=tm, fk_batch=fb, pass1_rounds=24, **kw)
         pb.enqueue()
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -86,16 +121,26 @@ def autotune(starts, goals, collision_context, *, max_nodes_per_query=None, samp
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / repeats
         solve = float(np.mean(pb.bufs.host_summary()["solved"] == 1))
-        row = {"num_new_samples": S, "threads": nt, "pass1_rounds": p1, "team_max": tm,
-               "plans_per_s": len(idx) / dt, "solve_rate": solve}
+        row = {"num_new_samples": S, "threads": nt, "team_max": tm, "fk_batch": fb,
+               "plans_per_s": len(idx) / dt, "ms": 1000.0 * dt, "solve_rate": solve}
         table.append(row)
         if log:
             log(f"autotune: {row}")
         del pb
-    base = [r["solve_rate"] for r in table if r["num_new_samples"] == DEFAULT_SAMPLES]
-    floor = max(base) if base else 0.0
-    ok = [r for r in table if r["solve_rate"] >= floor] or table
-    best = max(ok, key=lambda r: r["plans_per_s"])
+        return row
+
+    # stage 1: samples per round at the library's launch defaults
+    base = {S: run(S, 0, 0, 0) for S in samples}
+    floor = base[DEFAULT_SAMPLES]["solve_rate"] if DEFAULT_SAMPLES in base else 0.0
+    ok = [r for r in base.values() if r["solve_rate"] >= floor] or list(base.values())
+    S_best = max(ok, key=lambda r: r["plans_per_s"])["num_new_samples"]
+    # stage 2: the launch shape at that S (results do not depend on it)
+    for nt in threads:
+        for tm in team_max:
+            for fb in fk_batch:
+                run(S_best, nt, tm, fb)
+    cands = [r for r in table if r["num_new_samples"] == S_best]
+    best = max(cands, key=lambda r: r["plans_per_s"])
     return TuneResult(best["num_new_samples"],
-                      {"threads": best["threads"], "pass1_rounds": best["pass1_rounds"], "team_max": best["team_max"]},
-                      best["plans_per_s"], best["solve_rate"], table)
+                      {"threads": best["threads"], "team_max": best["team_max"], "fk_batch": best["fk_batch"]},
+                      best["plans_per_s"], best["solve_rate"], table, idx, [int(t) for t in tail])

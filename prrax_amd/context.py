@@ -32,6 +32,10 @@ REQUIRED_KEYS = (
     "sphere_radius",
 )
 WORLD_SPECS = (("world_spheres", 4), ("world_capsules", 7), ("world_boxes", 15), ("world_halfspaces", 6))
+# The optional coarse-checker keys (prrtc.py:76-110): key, row width (0 = a vector), dtype.
+COARSE_SPECS = (("coarse_sphere_link_idx", 0, np.int32), ("coarse_sphere_local", 3, np.float32),
+                ("coarse_sphere_radius", 0, np.float32), ("fine_link_offsets", 0, np.int32),
+                ("coarse_self_pairs", 2, np.int32), ("self_pair_ranges", 2, np.int32))
 MAX_JOINTS = 64
 MAX_DIM = 16
 
@@ -93,6 +97,15 @@ def check_context(cc: dict, dim: int) -> dict:
     for k, f in WORLD_SPECS:
         out[k] = _rows(cc, k, f, np.float32)
     out["self_pairs"] = _rows(cc, "self_pairs", 2, np.int32)
+    for k, f, dt in COARSE_SPECS:
+        if f:
+            out[k] = _rows(cc, k, f, dt)
+        else:
+            out[k] = as_numpy(cc[k], dt).reshape(-1) if k in cc and cc[k] is not None else np.zeros(0, dt)
+    nc = out["coarse_sphere_link_idx"].shape[0]
+    if out["coarse_sphere_local"].shape[0] != nc or out["coarse_sphere_radius"].shape[0] != nc:
+        raise ValueError("coarse_sphere_link_idx, coarse_sphere_local and coarse_sphere_radius disagree on the "
+                         "coarse sphere count")
     return out
 
 
@@ -129,6 +142,19 @@ def upload_robot(arrays: dict, device) -> DeviceRobot:
     return DeviceRobot(t, s, int(arrays["fk_parent_idx"].shape[0]), int(arrays["sphere_link_idx"].shape[0]))
 
 
+COUNT_KEYS = ("world_spheres_count", "world_capsules_count", "world_boxes_count", "world_halfspaces_count",
+              "self_pairs_count")
+
+
+def _coarse_fields(arrays: dict, device, t: dict) -> list:
+    """Device copies of the six coarse arrays (kept in ``t``) as the struct's pointer and count fields."""
+    for k, _, _ in COARSE_SPECS:
+        t[k] = torch.from_numpy(np.ascontiguousarray(arrays[k])).to(device)
+    return [*(_ptr(t[k]) for k, _, _ in COARSE_SPECS), int(arrays["coarse_sphere_link_idx"].shape[0]),
+            int(arrays["fine_link_offsets"].shape[0]), int(arrays["coarse_self_pairs"].shape[0]),
+            int(arrays["self_pair_ranges"].shape[0])]
+
+
 def upload_world(arrays: dict, device) -> DeviceWorld:
     keys = [k for k, _ in WORLD_SPECS] + ["self_pairs"]
     t = {k: torch.from_numpy(np.ascontiguousarray(arrays[k])).to(device) for k in keys}
@@ -136,13 +162,16 @@ def upload_world(arrays: dict, device) -> DeviceWorld:
         _ptr(t["world_spheres"]), _ptr(t["world_capsules"]), _ptr(t["world_boxes"]), _ptr(t["world_halfspaces"]),
         _ptr(t["self_pairs"]),
         *(int(arrays[k].shape[0]) for k in keys),
-        None,
+        None, None, None, None, None,
+        *_coarse_fields(arrays, device, t),
     )
     return DeviceWorld(t, s)
 
 
-def pack_worlds(worlds: list[dict], device) -> DeviceWorld:
-    """Per-query worlds padded to ``[B, max, F]`` with ``counts [B, 5]`` (prrtc.py:1009-1067)."""
+def pack_worlds(worlds: list[dict], device, coarse: dict | None = None) -> DeviceWorld:
+    """Per-query worlds padded to ``[B, max, F]`` plus the five per-query count vectors ``[B]``
+    (the reference's batch-ctx arguments, prrtc_planner.cu:1964-1974; padding at
+    prrtc.py:1009-1067).  ``coarse``: the checked arrays of the context the robot comes from."""
     B = len(worlds)
     keys = [(k, f, np.float32) for k, f in WORLD_SPECS] + [("self_pairs", 2, np.int32)]
     counts = np.zeros((B, 5), np.int32)
@@ -158,10 +187,13 @@ def pack_worlds(worlds: list[dict], device) -> DeviceWorld:
                 packed[i, : r.shape[0]] = r
         tens[k] = torch.from_numpy(packed).to(device)
         caps.append(cap)
-    tens["counts"] = torch.from_numpy(counts).to(device)
+    for col, k in enumerate(COUNT_KEYS):
+        tens[k] = torch.from_numpy(np.ascontiguousarray(counts[:, col])).to(device)
     s = _lib.World(
         _ptr(tens["world_spheres"]), _ptr(tens["world_capsules"]), _ptr(tens["world_boxes"]),
-        _ptr(tens["world_halfspaces"]), _ptr(tens["self_pairs"]), *caps, tens["counts"].data_ptr(),
+        _ptr(tens["world_halfspaces"]), _ptr(tens["self_pairs"]), *caps,
+        *(tens[k].data_ptr() for k in COUNT_KEYS),
+        *_coarse_fields(coarse if coarse is not None else worlds[0], device, tens),
     )
     return DeviceWorld(tens, s, per_query=True, counts=counts)
 

@@ -32,7 +32,8 @@ struct WorldDev {
     const int* sp;
     int n_ws, n_wc, n_wb, n_wh, n_sp;              // capacity (padded) counts
     long long s_ws, s_wc, s_wb, s_wh, s_sp;        // per-query strides in elements (0 = shared)
-    const int* counts;                             // [B,5] or null
+    const int* cnt[5];                             // per-query counts [B] each (spheres, capsules, boxes,
+                                                   // halfspaces, self pairs), null = the capacity
 };
 
 // Byte offsets of the model inside dynamic LDS, followed by the per-wave FK areas: wave w owns
@@ -195,11 +196,12 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
     float4* WH = reinterpret_cast<float4*>(lds + Lo.whs);
     const int nj = r.n_joints, nrs = r.n_spheres;
     int nws = w.n_ws, nwc = w.n_wc, nwb = w.n_wb, nwh = w.n_wh, nsp = w.n_sp;
-    if (w.counts) {
-        const int* c = w.counts + 5 * q;
-        nws = min(c[0], w.n_ws); nwc = min(c[1], w.n_wc); nwb = min(c[2], w.n_wb);
-        nwh = min(c[3], w.n_wh); nsp = min(c[4], w.n_sp);
-    }
+    // per-query counts, clamped to [0, capacity]
+    if (w.cnt[0]) nws = max(0, min(w.cnt[0][q], w.n_ws));
+    if (w.cnt[1]) nwc = max(0, min(w.cnt[1][q], w.n_wc));
+    if (w.cnt[2]) nwb = max(0, min(w.cnt[2][q], w.n_wb));
+    if (w.cnt[3]) nwh = max(0, min(w.cnt[3][q], w.n_wh));
+    if (w.cnt[4]) nsp = max(0, min(w.cnt[4][q], w.n_sp));
     const float* ws = w.ws + w.s_ws * q;
     const float* wc = w.wc + w.s_wc * q;
     const float* wb = w.wb + w.s_wb * q;

@@ -158,14 +158,15 @@ struct Model {
 // ---------------------------------------------------------------------------------- tests
 // Binary tests: the reference's "dist <= 0" (prrtc_planner.cu:250-304) without square roots.  A
 // sphere or capsule whose radius sum rr = r + R is >= 0 is hit iff d^2 <= rr^2, never when
-// rr < 0 (sqrt(d^2) - rr > 0); a box with sr >= 0 iff the squared outside distance is <= sr^2
-// (0 when the centre is inside), with sr < 0 iff the centre lies deeper than |sr| inside
-// (max q <= sr); a halfspace iff n.(c - p) <= sr.
+// rr < 0 (sqrt(d^2) - rr > 0): both at once as d^2 <= rr |rr| (|.| is a free operand modifier);
+// a box with sr >= 0 iff the squared outside distance is <= sr^2 (0 when the centre is inside),
+// with sr < 0 iff the centre lies deeper than |sr| inside (max q <= sr); a halfspace iff
+// n.(c - p) <= sr.
 __device__ __forceinline__ bool hit_sphere(float sx, float sy, float sz, float sr, const float4 o) {
     const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     const float rr = sr + o.w;
-    return rr >= 0.0f && d2 <= rr * rr;
+    return d2 <= rr * fabsf(rr);
 }
 
 __device__ __forceinline__ float capsule_d2(float sx, float sy, float sz, const CapsuleL& c) {
@@ -178,7 +179,7 @@ __device__ __forceinline__ float capsule_d2(float sx, float sy, float sz, const 
 
 __device__ __forceinline__ bool hit_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c) {
     const float rr = sr + c.r;
-    return rr >= 0.0f && capsule_d2(sx, sy, sz, c) <= rr * rr;
+    return capsule_d2(sx, sy, sz, c) <= rr * fabsf(rr);
 }
 
 __device__ __forceinline__ void box_q(float sx, float sy, float sz, const BoxL& b, float& q0, float& q1, float& q2) {
@@ -212,19 +213,19 @@ __device__ __forceinline__ bool hit_halfspace(float sx, float sy, float sz, floa
 }
 
 // Margin tests: the reference's "dist < margin" (prrtc_planner.cu:203-206, 314-387), also without
-// square roots.  With L = r + R + margin, a sphere or capsule is near iff L > 0 and d^2 < L^2; a
-// box (L = sr + margin) iff L > 0 and outside^2 < L^2 when the centre is outside (outside^2 > 0),
-// else iff min(max q, 0) < L; a halfspace iff n.(c - p) - sr < margin.
+// square roots.  With L = r + R + margin, a sphere or capsule is near iff L > 0 and d^2 < L^2
+// (d^2 < L |L|); a box (L = sr + margin) iff L > 0 and outside^2 < L^2 when the centre is outside
+// (outside^2 > 0), else iff min(max q, 0) < L; a halfspace iff n.(c - p) - sr < margin.
 __device__ __forceinline__ bool near_sphere(float sx, float sy, float sz, float sr, const float4 o, float margin) {
     const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     const float L = (sr + o.w) + margin;
-    return L > 0.0f && d2 < L * L;
+    return d2 < L * fabsf(L);
 }
 
 __device__ __forceinline__ bool near_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c, float margin) {
     const float L = (sr + c.r) + margin;
-    return L > 0.0f && capsule_d2(sx, sy, sz, c) < L * L;
+    return capsule_d2(sx, sy, sz, c) < L * fabsf(L);
 }
 
 __device__ __forceinline__ bool near_box(float sx, float sy, float sz, float sr, const BoxL& b, float margin) {
@@ -538,7 +539,9 @@ __device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4
 }
 
 // Fine test of robot sphere q (world x, y, z, r) against obstacle o: hit (binary) or distance
-// below the margin (MARGIN), exactly as the flat check computes it.
+// below the margin (MARGIN), exactly as the flat check computes it.  The checks below take the
+// mode at run time (M.mode == 1: margin; it is wave-uniform) and branch to one of the two
+// instances at each fine-test site, so each check exists once per kernel.
 template <bool MARGIN>
 __device__ __forceinline__ bool fine_test(const Model& M, int o, const float4 q, unsigned& fl) {
     if (o < M.n_wh) {
@@ -560,6 +563,10 @@ __device__ __forceinline__ bool fine_test(const Model& M, int o, const float4 q,
     o -= M.n_wc;
     fl += kFlopsSS;
     return MARGIN ? near_sphere(q.x, q.y, q.z, q.w, M.wsph[o], M.margin) : hit_sphere(q.x, q.y, q.z, q.w, M.wsph[o]);
+}
+
+__device__ __forceinline__ bool fine_test_mode(const Model& M, bool mg, int o, const float4 q, unsigned& fl) {
+    return mg ? fine_test<true>(M, o, q, fl) : fine_test<false>(M, o, q, fl);
 }
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
@@ -598,20 +605,19 @@ __device__ __forceinline__ bool self_pairs_2(const Model& M, const float4* FP, i
     return h;
 }
 
-// The two checks below are kept out of line (noinline): inlined into the planner's round loop
-// they push its 16-dimension team kernel past 256 VGPRs, the compiler then outlines the whole
-// round loop with a copy of the kernel arguments in private memory (about 1.5 KB per lane), and a
-// cooperative launch that keeps every work-group resident needs that much scratch for every lane
-// of the GPU at once.  Out of line, each check has its own small frame.
-template <bool MARGIN>
-__device__ __attribute__((noinline)) bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
+__device__ __forceinline__ bool self_pairs_mode(const Model& M, bool mg, const float4* FP, int k0, int lane) {
+    return mg ? self_pairs_2<true>(M, FP, k0, lane) : self_pairs_2<false>(M, FP, k0, lane);
+}
+
+__device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
     char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
     float4* FP = reinterpret_cast<float4*>(area);          // robot spheres, world frame
     float4* CW = FP + M.n_spheres;                          // group bounding spheres, world frame
     int* Q = reinterpret_cast<int*>(CW + M.n_joints);       // refinement queue
     const int ng = M.n_groups, nrs = M.n_spheres;
-    const float cut = MARGIN ? M.margin : 0.0f;
+    const bool mg = M.mode == 1;
+    const float cut = mg ? M.margin : 0.0f;
     unsigned lf = 0;  // this lane's flops of fine tests
     auto done = [&](bool v) {
         flops += wave_sum_u32(lf);
@@ -659,7 +665,7 @@ __device__ __attribute__((noinline)) bool wave_culled(const Model& M, const floa
         for (int e = lane; e < qn && !h; e += kWave) {
             const int o = Q[e] >> 6, g = Q[e] & 63;
             for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
-                if (fine_test<MARGIN>(M, o, FP[M.gmem[m]], lf)) {
+                if (fine_test_mode(M, mg, o, FP[M.gmem[m]], lf)) {
                     h = true;
                     break;
                 }
@@ -744,7 +750,7 @@ __device__ __attribute__((noinline)) bool wave_culled(const Model& M, const floa
     }
     flops += coarse_flops_of(M, total, ng) + static_cast<unsigned long long>(nws) * ng * kFlopsSS;
     for (int p0 = 0; p0 < M.n_pairs; p0 += 2 * kWave) {
-        const bool h = self_pairs_2<MARGIN>(M, FP, p0, lane);
+        const bool h = self_pairs_mode(M, mg, FP, p0, lane);
         flops += static_cast<unsigned long long>(min(2 * kWave, M.n_pairs - p0)) * kFlopsPair;
         if (wave_any(h)) return done(true);
     }
@@ -764,8 +770,7 @@ __device__ __attribute__((noinline)) bool wave_culled(const Model& M, const floa
 // configurations are then tested together (wave_culled's coarse and fine tests, 64 pairs per wave
 // step, stopping at the first step with a hit), and, if none hits the world, the self pairs of
 // each configuration in order.  Whether any configuration collides is wave_culled's verdict.
-template <bool MARGIN>
-__device__ __attribute__((noinline)) int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
+__device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
     char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
     float4* FP = reinterpret_cast<float4*>(area);          // robot spheres (per configuration)
@@ -774,7 +779,8 @@ __device__ __attribute__((noinline)) int wave_unit_check(const Model& M, const f
     int* LU = Q + kCullQ;                                   // candidate pairs of the sub-unit
     float4* CU = FP;  // unit spheres (n_groups <= n_spheres); free until the first transform below
     const int ng = M.n_groups, nrs = M.n_spheres;
-    const float cut = MARGIN ? M.margin : 0.0f;
+    const bool mg = M.mode == 1;
+    const float cut = mg ? M.margin : 0.0f;
     unsigned lf = 0;
     auto done = [&](int v) {
         flops += wave_sum_u32(lf);
@@ -896,7 +902,7 @@ __device__ __attribute__((noinline)) int wave_unit_check(const Model& M, const f
                 const float4 sp = M.spheres[M.gmem[m]];
                 const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
                 lf += kFlopsXf;
-                if (fine_test<MARGIN>(M, o, make_float4(p.x, p.y, p.z, sp.w), lf)) {
+                if (fine_test_mode(M, mg, o, make_float4(p.x, p.y, p.z, sp.w), lf)) {
                     hm |= 1u << u;
                     break;
                 }
@@ -938,7 +944,7 @@ __device__ __attribute__((noinline)) int wave_unit_check(const Model& M, const f
             }
             wave_sync();
             for (int p0 = 0; p0 < M.n_pairs; p0 += 2 * kWave) {
-                const bool h = self_pairs_2<MARGIN>(M, FP, p0, lane);
+                const bool h = self_pairs_mode(M, mg, FP, p0, lane);
                 flops += static_cast<unsigned long long>(min(2 * kWave, M.n_pairs - p0)) * kFlopsPair;
                 if (wave_any(h)) return done(u + 1);
             }
@@ -953,16 +959,14 @@ __device__ __attribute__((noinline)) int wave_unit_check(const Model& M, const f
 __device__ __forceinline__ bool wave_in_collision(const Model& M, const float* T, int ts, unsigned long long& flops) {
     if (!M.enabled) return false;
     flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
-    if (M.mode == 1) return wave_culled<true>(M, T, ts, flops);
-    return wave_culled<false>(M, T, ts, flops);
+    return wave_culled(M, T, ts, flops);
 }
 
 // wave_unit_check with the mode dispatch of wave_in_collision (nu >= 1 configurations at T + c).
 __device__ __forceinline__ int wave_unit_in_collision(const Model& M, const float* T, int ts, int nu,
                                                       unsigned long long& flops) {
     if (!M.enabled) return 0;
-    if (M.mode == 1) return wave_unit_check<true>(M, T, ts, nu, flops);
-    return wave_unit_check<false>(M, T, ts, nu, flops);
+    return wave_unit_check(M, T, ts, nu, flops);
 }
 
 }  // namespace prrtc

@@ -48,7 +48,11 @@ WorldDev world_dev(const prrtc_world& w, bool per_query) {
     d.n_sp = w.n_self_pairs;
     d.s_ws = per_query ? 4LL * d.n_ws : 0; d.s_wc = per_query ? 7LL * d.n_wc : 0; d.s_wb = per_query ? 15LL * d.n_wb : 0;
     d.s_wh = per_query ? 6LL * d.n_wh : 0; d.s_sp = per_query ? 2LL * d.n_sp : 0;
-    d.counts = per_query ? w.counts : nullptr;
+    d.cnt[0] = per_query ? w.world_spheres_count : nullptr;
+    d.cnt[1] = per_query ? w.world_capsules_count : nullptr;
+    d.cnt[2] = per_query ? w.world_boxes_count : nullptr;
+    d.cnt[3] = per_query ? w.world_halfspaces_count : nullptr;
+    d.cnt[4] = per_query ? w.self_pairs_count : nullptr;
     return d;
 }
 
@@ -75,6 +79,16 @@ int check_world(const prrtc_world* w) {
         (w->n_world_boxes && !w->world_boxes) || (w->n_world_halfspaces && !w->world_halfspaces) ||
         (w->n_self_pairs && !w->self_pairs))
         return fail_with(PRRTC_E_INVALID, "world array null with nonzero count");
+    // The coarse-checker arrays: sizes consistent, pointers present where sizes are nonzero.
+    if (w->n_coarse_spheres < 0 || w->n_fine_link_offsets < 0 || w->n_coarse_self_pairs < 0 ||
+        w->n_self_pair_ranges < 0)
+        return fail_with(PRRTC_E_INVALID, "negative coarse-array count");
+    if (w->n_coarse_spheres &&
+        (!w->coarse_sphere_link_idx || !w->coarse_sphere_local || !w->coarse_sphere_radius))
+        return fail_with(PRRTC_E_INVALID, "coarse sphere arrays null with nonzero count");
+    if ((w->n_fine_link_offsets && !w->fine_link_offsets) || (w->n_coarse_self_pairs && !w->coarse_self_pairs) ||
+        (w->n_self_pair_ranges && !w->self_pair_ranges))
+        return fail_with(PRRTC_E_INVALID, "coarse array null with nonzero count");
     return PRRTC_OK;
 }
 

@@ -53,6 +53,8 @@ enum { QS_ROUND, QS_SIZE_A, QS_SIZE_B, QS_TIME, QS_COUNT = 8 };
 // Control words ([kCtl] ints).  CTL_NC + i counts continue list i (three lists rotate over the
 // epochs of pass 2: one read, one filled, one cleared).
 enum { CTL_QUEUE, CTL_ERROR, CTL_NC, kCtl = 16 };
+// uint64 statistics at the end of the workspace (layout in prrtc_amd.h, prrtc_planner_stats_offset).
+constexpr int kStats = 32;
 
 struct PlanArgs {
     int max_iterations;
@@ -97,7 +99,7 @@ struct PlanArgs {
     int* ctl;                    // [kCtl]
     int* cont;                   // [3][B] continue lists
     unsigned* tsync;             // team barrier lines (make_team); the last team is the grid
-    unsigned long long* stats;   // [4]
+    unsigned long long* stats;   // [kStats]
     // scheduling
     int pass1_rounds;            // round budget of pass 1 (0: pass 1 only initialises)
     int wmax;                    // team size cap
@@ -535,6 +537,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             __syncthreads();
             nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
                                t ? coh_b : coh_a);
+            if (tid == 0) {  // node bytes of this tile pass (no return value: the atomics do not wait)
+                global_add(&a.stats[16], 4ull * static_cast<unsigned long long>(dim) * static_cast<unsigned>(hi - lo));
+                global_add(&a.stats[17], 1ull);
+            }
             if (tid < nc) gmin64(&keys[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
             __syncthreads();
         }
@@ -1008,6 +1014,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         const int base = pass == 1 ? 4 : 10;  // pass 1 / pass 2 phase times
         for (int k = 0; k < 6; ++k) global_add(&a.stats[base + k], ph[k]);
     }
+
     const int st = sc[SC_ERR] ? -1 : sc[SC_STATUS];
     const unsigned long long t1 = wall_clock64();
     const float dt = static_cast<float>(static_cast<double>(t1 - t0) / a.wall_khz);
@@ -1227,7 +1234,7 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
 __global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, unsigned long long shape) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < kCtl) ctl[t] = 0;
-    if (t < 16) stats[t] = (t == 3) ? shape : 0ull;
+    if (t < kStats) stats[t] = (t == 3) ? shape : 0ull;
     const int lines = static_cast<int>(sync_words() / kSyncLine);
     for (int i = t; i < lines; i += gridDim.x * blockDim.x) tsync[static_cast<size_t>(i) * kSyncLine] = 0u;
 }
@@ -1247,7 +1254,7 @@ WsLayout ws_layout(int B, int ncap, int S) {
     w.cont = o; o = align16(o + sizeof(int) * 3 * static_cast<size_t>(B));
     o = (o + 255) & ~static_cast<size_t>(255);  // barrier lines start on a 256-byte boundary
     w.tsync = o; o = align16(o + sizeof(unsigned) * sync_words());
-    w.stats = o; o = align16(o + 16 * sizeof(unsigned long long));
+    w.stats = o; o = align16(o + kStats * sizeof(unsigned long long));
     w.total = o + 256;
     return w;
 }
@@ -1332,7 +1339,9 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     if (rc) return rc;
     rc = check_world(world);
     if (rc) return rc;
-    if (world_per_query && !world->counts) return fail_with(PRRTC_E_INVALID, "per-query world needs counts");
+    if (world_per_query && !(world->world_spheres_count && world->world_capsules_count && world->world_boxes_count &&
+                             world->world_halfspaces_count && world->self_pairs_count))
+        return fail_with(PRRTC_E_INVALID, "per-query worlds need the five count vectors");
     const int dim = at->dim, S = at->num_new_samples, ncap = at->max_nodes;
     const WsLayout wl = ws_layout(B, ncap, S);
     if (!workspace || workspace_bytes < wl.total) return fail_with(PRRTC_E_WORKSPACE, "workspace too small");
@@ -1373,10 +1382,14 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
     // LDS: the FK batch (16 configurations per wave, or 8 when that lets more work-groups stay
     // resident, as for large robots), shrinking the work-group if even that does not fit.
     const bool small = dim <= 8;
+    const int fk_batch = launch ? launch->fk_batch : 0;
+    if (fk_batch != 0 && fk_batch != kFB && fk_batch != kFB / 2)
+        return fail_with(PRRTC_E_INVALID, "fk_batch must be 0, 8 or 16");
     size_t lds = 0;
     int per_cu = 0;
     for (;;) {
         for (const int fbp : {kFBP, kFB / 2 + 1}) {
+            if (fk_batch != 0 && fbp != fk_batch + 1) continue;
             PlanArgs t = a;
             t.ml = model_layout(robot->n_joints, robot->n_robot_spheres, world->n_self_pairs, world->n_world_spheres,
                                 world->n_world_capsules, world->n_world_boxes, world->n_world_halfspaces, nt / kWave,

@@ -127,10 +127,12 @@ class PlanBuffers:
     PHASES = ("sample_nn", "extend_check", "commit", "connect_nn", "connect_walk", "connect_commit")
 
     def stats(self) -> dict:
-        """Counters of the last launch: configs checked, rounds, algorithmic flops, and device
-        wall-clock ticks per round phase for the per-query pass and the team pass."""
-        raw = self.workspace[self.stats_offset:self.stats_offset + 128].cpu().numpy().view(np.uint64)
-        out = {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2])}
+        """Counters of the last launch: configs checked, rounds, algorithmic flops and NN scan
+        bytes, and device wall-clock ticks per round phase for the per-query pass and the team
+        pass."""
+        raw = self.workspace[self.stats_offset:self.stats_offset + 256].cpu().numpy().view(np.uint64)
+        out = {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2]), "nn_bytes": int(raw[16]),
+               "nn_tile_passes": int(raw[17])}
         shape = int(raw[3])
         out["launch"] = {"slots": shape & 0xFFFF, "threads": (shape >> 16) & 0xFFFF,
                          "pass1_rounds": (shape >> 32) & 0xFFFF, "team_max": (shape >> 48) & 0xFFFF}
@@ -384,6 +386,7 @@ def prepare_batch(
     pass1_rounds: int = -1,
     team_max: int = 0,
     epoch_rounds: int = 0,
+    fk_batch: int = 0,
 ) -> PreparedBatch:
     """Validates and uploads a batch; ``PreparedBatch.enqueue()`` then runs it (repeatably).
 
@@ -444,7 +447,7 @@ def prepare_batch(
                 except ValueError as e:
                     raise ValueError(f"collision_context[{i}]: {e}") from None
         robot = upload_robot(base, dev)
-        world = pack_worlds(worlds, dev)
+        world = pack_worlds(worlds, dev, coarse=base)
     else:
         if collision_context is None and not allow_unsafe_no_collision:
             raise ValueError(
@@ -482,7 +485,7 @@ def prepare_batch(
         robot, world, torch.from_numpy(np.ascontiguousarray(starts)).to(dev),
         torch.from_numpy(np.ascontiguousarray(goals)).to(dev),
         None if gcounts is None else torch.from_numpy(gcounts).to(dev), lo, hi, mnq, sd, od, attrs, bufs,
-        _lib.Launch(int(threads), int(slots), int(pass1_rounds), int(team_max), int(epoch_rounds)))
+        _lib.Launch(int(threads), int(slots), int(pass1_rounds), int(team_max), int(epoch_rounds), int(fk_batch)))
 
 
 def prrtc_plan_batch(
@@ -516,6 +519,7 @@ def prrtc_plan_batch(
     pass1_rounds: int = -1,
     team_max: int = 0,
     epoch_rounds: int = 0,
+    fk_batch: int = 0,
 ) -> list[PRRTCResult]:
     """Plan a batch of independent queries in one persistent launch (prrtc.py:882-1434).
 
@@ -531,7 +535,7 @@ def prrtc_plan_batch(
         dd_min_radius=dd_min_radius, min_vals=min_vals, max_vals=max_vals, collision_checker=collision_checker,
         collision_margin=collision_margin, max_nodes_per_query=max_nodes_per_query, seeds=seeds, order=order,
         path_capacity=path_capacity, threads=threads, slots=slots, pass1_rounds=pass1_rounds, team_max=team_max,
-        epoch_rounds=epoch_rounds)
+        epoch_rounds=epoch_rounds, fk_batch=fk_batch)
     pb.enqueue()
     return pb.bufs.results()
 
