@@ -236,13 +236,16 @@ int prrtc_sphere_cost(prrtc_stream_t stream, const float* configs, int32_t n, in
 /* Workspace bytes of prrtc_refine_paths. */
 size_t prrtc_refine_workspace_bytes(int32_t n_paths, int32_t n_waypoints, int32_t dim);
 
-/* Path refinement (no reference counterpart): `iterations` Jacobi gradient steps on the
- * interior waypoints of paths [n_paths, n_waypoints, dim] (updated in place) of
- *   sum of prrtc_sphere_cost over all waypoints (world->world_spheres) + smooth_weight * sum |q_{k+1} - q_k|^2,
- * each candidate clamped to [min_vals, max_vals] (both NULL = no clamp) and kept only where the
- * exact predicate (collision_mode/margin against the whole world, as prrtc_check_configs) says
- * free; end points never move.  out_cost [iterations + 1] (may be NULL): the total cost before
- * each step and after the last. */
+/* Path refinement (no reference counterpart): `iterations` gradient steps on the interior
+ * waypoints of paths [n_paths, n_waypoints, dim] (updated in place) of
+ *   sum of prrtc_sphere_cost over all waypoints (world->world_spheres) + smooth_weight * sum |q_{k+1} - q_k|^2.
+ * Each step computes every waypoint's candidate q - step * gradient (clamped to [min_vals,
+ * max_vals]; both NULL = no clamp), then moves the even interior waypoints and after them the
+ * odd ones: a waypoint takes its candidate only if the candidate and 15 interior points of each
+ * edge to its neighbours (t = j/16) pass the exact predicate (collision_mode/margin against the
+ * whole world, as prrtc_check_configs), so every edge of a refined path is checked.  End points
+ * never move.  out_cost [iterations + 1] (may be NULL): the total cost before each step and
+ * after the last. */
 int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int32_t n_waypoints, int32_t dim,
                        const float* min_vals, const float* max_vals, const prrtc_robot* robot,
                        const prrtc_world* world, int32_t collision_mode, float collision_margin, float beta,

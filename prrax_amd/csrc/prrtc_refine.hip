@@ -240,11 +240,61 @@ __global__ void refine_update_kernel(const float* paths, const float* grad, int 
     cand[t] = c;
 }
 
-// Keep a candidate waypoint only where the exact predicate found it free.
-__global__ void refine_accept_kernel(float* paths, const float* cand, const uint8_t* hit, int n, int dim) {
+// Acceptance of the moves.  A waypoint k may move to its candidate only if the candidate and
+// kEdgeRes - 1 interior points of each of its two edges (to the current neighbours, at t = j /
+// kEdgeRes) pass the exact predicate, so a refined path is free wherever the path it came from
+// was checked.  Waypoints of one parity move together (their neighbours, of the other parity,
+// stay put), the even ones first, then the odd ones against the moved even ones.
+constexpr int kEdgeRes = 16;
+constexpr int kMovePoints = 2 * kEdgeRes - 1;  // the candidate + both edges' interior points
+
+// Interior waypoints of parity par per path: k = 1 + par + 2 j, k <= L - 2.
+__host__ __device__ inline int parity_slots(int L, int par) { return (L - 2 - par + 1) / 2; }
+
+// Configurations that decide the moves of parity par: [P][slots][kMovePoints][dim], slots =
+// parity_slots(L, 0) (the larger count; unused slots of the odd phase hold the path's start).
+__global__ void refine_move_points_kernel(const float* paths, const float* cand, int P, int L, int dim, int par,
+                                          int slots, float* out) {
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= static_cast<long long>(n) * dim) return;
-    if (!hit[t / dim]) paths[t] = cand[t];
+    const long long total = static_cast<long long>(P) * slots * kMovePoints;
+    if (t >= total) return;
+    const int c = static_cast<int>(t % kMovePoints);
+    const int j = static_cast<int>((t / kMovePoints) % slots);
+    const int p = static_cast<int>(t / (static_cast<long long>(kMovePoints) * slots));
+    const int k = 1 + par + 2 * j;
+    float* o = out + t * dim;
+    const float* row = paths + static_cast<size_t>(p) * L * dim;
+    if (k > L - 2) {
+        for (int d = 0; d < dim; ++d) o[d] = row[d];
+        return;
+    }
+    const float* ck = cand + (static_cast<size_t>(p) * L + k) * dim;
+    const float* a = row + static_cast<size_t>(k - 1) * dim;
+    const float* b = row + static_cast<size_t>(k + 1) * dim;
+    if (c == 0) {
+        for (int d = 0; d < dim; ++d) o[d] = ck[d];
+    } else if (c < kEdgeRes) {  // edge (a, candidate) at t = c / kEdgeRes
+        const float tf = static_cast<float>(c) / static_cast<float>(kEdgeRes);
+        for (int d = 0; d < dim; ++d) o[d] = a[d] + tf * (ck[d] - a[d]);
+    } else {  // edge (candidate, b) at t = (c - kEdgeRes + 1) / kEdgeRes
+        const float tf = static_cast<float>(c - kEdgeRes + 1) / static_cast<float>(kEdgeRes);
+        for (int d = 0; d < dim; ++d) o[d] = ck[d] + tf * (b[d] - ck[d]);
+    }
+}
+
+// Moves waypoint k of parity par to its candidate where none of its kMovePoints points collides.
+__global__ void refine_accept_kernel(float* paths, const float* cand, const uint8_t* hit, int P, int L, int dim,
+                                     int par, int slots) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(P) * slots) return;
+    const int j = static_cast<int>(t % slots), p = static_cast<int>(t / slots);
+    const int k = 1 + par + 2 * j;
+    if (k > L - 2) return;
+    const uint8_t* h = hit + t * kMovePoints;
+    for (int c = 0; c < kMovePoints; ++c)
+        if (h[c]) return;
+    const size_t off = (static_cast<size_t>(p) * L + k) * dim;
+    for (int d = 0; d < dim; ++d) paths[off + d] = cand[off + d];
 }
 
 // Total cost of all paths: sum of the per-waypoint collision costs plus the smoothness term.
@@ -335,16 +385,20 @@ hipError_t launch_cost(const CostPlan& cp, hipStream_t s) {
 }
 
 struct RefineWs {
-    size_t cand, grad, cost, hit, total;
+    size_t cand, grad, cost, pts, hit, total;
+    long long n_pts;
 };
 
-RefineWs refine_ws(long long n, int dim) {
+RefineWs refine_ws(int P, int L, int dim) {
     RefineWs w;
+    const long long n = static_cast<long long>(P) * L;
+    w.n_pts = static_cast<long long>(P) * std::max(parity_slots(L, 0), 0) * kMovePoints;
     size_t o = 0;
     w.cand = o; o = align16(o + sizeof(float) * n * dim);
     w.grad = o; o = align16(o + sizeof(float) * n * dim);
     w.cost = o; o = align16(o + sizeof(float) * n);
-    w.hit = o; o = align16(o + static_cast<size_t>(n));
+    w.pts = o; o = align16(o + sizeof(float) * static_cast<size_t>(w.n_pts) * dim);
+    w.hit = o; o = align16(o + static_cast<size_t>(w.n_pts));
     w.total = o + 256;
     return w;
 }
@@ -373,8 +427,8 @@ int prrtc_sphere_cost(prrtc_stream_t stream, const float* configs, int32_t n, in
 }
 
 size_t prrtc_refine_workspace_bytes(int32_t n_paths, int32_t n_waypoints, int32_t dim) {
-    if (n_paths < 0 || n_waypoints < 0 || dim < 1) return 0;
-    return refine_ws(static_cast<long long>(n_paths) * n_waypoints, dim).total;
+    if (n_paths < 0 || n_waypoints < 2 || dim < 1) return 0;
+    return refine_ws(n_paths, n_waypoints, dim).total;
 }
 
 int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int32_t n_waypoints, int32_t dim,
@@ -396,13 +450,16 @@ int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int
     if (rc) return rc;
     if (n == 0) return PRRTC_OK;
     if (!paths) return fail_with(PRRTC_E_INVALID, "paths is null");
-    const RefineWs wl = refine_ws(n, dim);
+    const RefineWs wl = refine_ws(n_paths, n_waypoints, dim);
     if (!workspace || workspace_bytes < wl.total) return fail_with(PRRTC_E_WORKSPACE, "workspace too small");
+    if (wl.n_pts > (1ll << 31) - 1) return fail_with(PRRTC_E_INVALID, "too many move checks per step");
     char* ws = static_cast<char*>(workspace);
     float* cand = reinterpret_cast<float*>(ws + wl.cand);
     float* grad = reinterpret_cast<float*>(ws + wl.grad);
     float* cost = reinterpret_cast<float*>(ws + wl.cost);
+    float* pts = reinterpret_cast<float*>(ws + wl.pts);
     uint8_t* hit = reinterpret_cast<uint8_t*>(ws + wl.hit);
+    const int slots = parity_slots(n_waypoints, 0);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     cp.a.configs = paths;
     cp.a.out_cost = cost;
@@ -421,13 +478,20 @@ int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int
         if (last) break;
         hipLaunchKernelGGL(refine_update_kernel, dim3(eblocks), dim3(256), 0, s, paths, grad, n_paths, n_waypoints,
                            dim, step, smooth_weight, min_vals, max_vals, cand);
-        rc = prrtc_check_configs(stream, cand, static_cast<int32_t>(n), dim, robot, world, collision_mode,
-                                 collision_margin, hit, nullptr, nullptr, work_flops);
-        if (rc) return rc;
-        hipLaunchKernelGGL(refine_accept_kernel, dim3(eblocks), dim3(256), 0, s, paths, cand, hit,
-                           static_cast<int>(n), dim);
-        e = hipGetLastError();
-        if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+        if (slots <= 0) continue;  // two waypoints: nothing moves
+        for (int par = 0; par < 2; ++par) {
+            const int pblocks = static_cast<int>((wl.n_pts + 255) / 256);
+            hipLaunchKernelGGL(refine_move_points_kernel, dim3(pblocks), dim3(256), 0, s, paths, cand, n_paths,
+                               n_waypoints, dim, par, slots, pts);
+            rc = prrtc_check_configs(stream, pts, static_cast<int32_t>(wl.n_pts), dim, robot, world, collision_mode,
+                                     collision_margin, hit, nullptr, nullptr, work_flops);
+            if (rc) return rc;
+            const long long moves = static_cast<long long>(n_paths) * slots;
+            hipLaunchKernelGGL(refine_accept_kernel, dim3(static_cast<int>((moves + 255) / 256)), dim3(256), 0, s,
+                               paths, cand, hit, n_paths, n_waypoints, dim, par, slots);
+            e = hipGetLastError();
+            if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
+        }
     }
     return PRRTC_OK;
 }

@@ -117,9 +117,47 @@ def test_refine_paths_keeps_ends_and_validity():
     ctx = O.Context(cc, robot.dim)
     moved = np.any(out != P, axis=2)
     hits = O.in_collision(ctx, out.reshape(-1, robot.dim)).reshape(out.shape[:2])
-    # every moved waypoint passed the exact predicate (the oracle's, bit-exact with the GPU's)
+    # every moved waypoint passed the exact predicate (the oracle's, bit-exact with the GPU's) ...
     assert not np.any(hits & moved)
+    # ... and so did the interior points t = j/16 of every edge with a moved end
+    t = (np.arange(1, 16, dtype=np.float32) / np.float32(16))[:, None]
+    for p in range(out.shape[0]):
+        for k in range(out.shape[1] - 1):
+            if moved[p, k] or moved[p, k + 1]:
+                a, b = out[p, k], out[p, k + 1]
+                assert not O.in_collision(ctx, (a + t * (b - a)).astype(np.float32)).any(), (p, k)
     assert np.all(out >= robot.lower - 1e-6) and np.all(out <= robot.upper + 1e-6)
+
+
+def _planar_slider(world_spheres) -> dict:
+    """A point-like robot sphere (radius 0.01) moved by two prismatic joints along x and y."""
+    return {
+        "fk_twists": np.array([[1, 0, 0, 0, 0, 0], [0, 1, 0, 0, 0, 0]], np.float32),
+        "fk_parent_tf": np.array([[1, 0, 0, 0, 0, 0, 0]] * 2, np.float32),
+        "fk_parent_idx": np.array([-1, 0], np.int32), "fk_act_idx": np.array([0, 1], np.int32),
+        "fk_mimic_mul": np.ones(2, np.float32), "fk_mimic_off": np.zeros(2, np.float32),
+        "fk_mimic_act_idx": np.array([-1, -1], np.int32), "fk_topo_inv": np.array([0, 1], np.int32),
+        "sphere_link_idx": np.array([1], np.int32), "sphere_local": np.zeros((1, 3), np.float32),
+        "sphere_radius": np.array([0.01], np.float32), "world_spheres": np.asarray(world_spheres, np.float32),
+    }
+
+
+@pytest.mark.parametrize("obstacle,moves", [((0.25, -0.08, 0.0, 0.03), False), ((0.25, -0.5, 0.0, 0.03), True)])
+def test_refine_rejects_a_move_whose_edge_crosses_an_obstacle(obstacle, moves):
+    """Path (0, 0) -> (0.5, 1) -> (1, 0): the smoothness term alone (the obstacle is far from every
+    waypoint) pulls the middle waypoint to (0.5, -0.2), a free point.  With a thin obstacle at
+    (0.25, -0.08) the new first edge passes 0.02 from its centre, so the move must be refused;
+    with the obstacle at (0.25, -0.5) it is taken."""
+    cc = _planar_slider([obstacle])
+    P = np.array([[[0.0, 0.0], [0.5, 1.0], [1.0, 0.0]]], np.float32)
+    ctx = O.Context(cc, 2)
+    assert not O.in_collision(ctx, np.array([[0.5, -0.2], [0.0, 0.0], [1.0, 0.0]], np.float32)).any()
+    out, _ = refine_paths(P, cc, iterations=1, step=3.0, smooth_weight=0.1)
+    got = out.cpu().numpy()[0, 1]
+    if moves:
+        np.testing.assert_allclose(got, [0.5, -0.2], atol=1e-6)
+    else:
+        np.testing.assert_array_equal(got, [0.5, 1.0])
 
 
 def test_refine_zero_iterations_is_identity():
