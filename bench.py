@@ -386,7 +386,7 @@ def cpu_single_query_leg(pool, n: int) -> dict:
     jobs = []
     for qi in range(n):
         rng = np.random.default_rng(scenes.query_seed(1, qi))
-        sg = scenes.sample_free(rng, sc.robot, 2, lambda q: min_distance(cc, np.asarray(q)[None])[0] > 1e-5)
+        sg = scenes.sample_free(rng, sc.robot, 2, lambda q: min_distance(cc, q) > 1e-5)
         jobs.append((cc, sg[0], sg[1:], dict(max_iterations=10_000, step_size=0.5, num_new_samples=128, granularity=16,
                                              max_nodes=16384, min_vals=sc.robot.lower, max_vals=sc.robot.upper,
                                              seed=qi), 60.0))

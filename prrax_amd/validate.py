@@ -158,6 +158,27 @@ def min_distance(cc: dict, qs, cap: float | None = None) -> np.ndarray:
     return best
 
 
+def self_pair_distances(cc: dict, q) -> np.ndarray:
+    """Signed distance of every self pair of the context at one configuration, in the order of
+    ``self_pairs`` (+inf for a pair with an index out of range or a sphere on no joint)."""
+    sp = np.asarray(cc.get("self_pairs", np.zeros((0, 2))), np.int64).reshape(-1, 2)
+    out = np.full(sp.shape[0], np.inf)
+    links = np.asarray(cc["sphere_link_idx"]).reshape(-1)
+    nj = np.asarray(cc["fk_parent_idx"]).reshape(-1).shape[0]
+    if sp.shape[0] == 0 or links.shape[0] == 0 or nj == 0:
+        return out
+    P, r = sphere_centres(cc, np.asarray(q, np.float64).reshape(1, -1))
+    ok = (links >= 0) & (links < nj)
+    pos = np.full(links.shape[0], -1)
+    pos[ok] = np.arange(int(ok.sum()))
+    good = (sp >= 0).all(1) & (sp < links.shape[0]).all(1)
+    i = np.where(good, pos[np.clip(sp[:, 0], 0, links.shape[0] - 1)], -1)
+    j = np.where(good, pos[np.clip(sp[:, 1], 0, links.shape[0] - 1)], -1)
+    keep = (i >= 0) & (j >= 0)
+    out[keep] = np.linalg.norm(P[0, i[keep]] - P[0, j[keep]], axis=-1) - r[i[keep]] - r[j[keep]]
+    return out
+
+
 def path_points(path: np.ndarray, resolution: int = 16) -> np.ndarray:
     """Every point a + (k / resolution)(b - a), k = 0 .. resolution, of every edge of a path."""
     p = np.asarray(path, np.float64)
