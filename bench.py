@@ -33,10 +33,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (datasheet)
 PEAK_HBM_GBS = 8000.0
-# HBM bytes of one batched step of the default 1024-query workload (plan_pass1 + plan_pass2) at
-# the autotuned shape: rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) plus
-# WRITE_SIZE (x 1.0), from the profile named below (measured by its own runs, not by this one).
-TRAFFIC = {"bytes_per_step": None, "source": None}
+# HBM bytes of one batched step of the 1024-query workload (plan_pass1 + plan_pass2): rocprofv3
+# FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) plus WRITE_SIZE (x 1.0), measured by the
+# profile runs named in "source", not by this run.  Reported only for the launch shape they ran.
+TRAFFIC = {"bytes_per_step": 13.28e9,
+           "source": "profiles/plan_batched_1024q_hbm_counters.md: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
+                     "of the same workload at this launch shape (337.4 ms step)",
+           "shape": {"num_new_samples": 512, "slots": 256, "threads": 512, "pass1_rounds": 24, "team_max": 512,
+                     "fk_batch": 16}}
 
 METRIC = "plans/sec + p50 ms/plan, batched vs per-query at equal solve rate; FK+CC cfg/s"
 
@@ -549,6 +553,8 @@ def main():
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
+    shape_now = dict(stats.get("launch", {}), num_new_samples=kw["num_new_samples"])
+    same_shape = args.queries == 1024 and all(shape_now.get(k) == v for k, v in TRAFFIC["shape"].items())
     nn_gbs = stats["nn_bytes"] / (kernel_ms / 1000.0) / 1e9
     out = {
         "metric": METRIC,
@@ -588,7 +594,9 @@ def main():
         "dual_arm": dual,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-                     "traffic": TRAFFIC["bytes_per_step"], "traffic_source": TRAFFIC["source"],
+                     "traffic": TRAFFIC["bytes_per_step"] if same_shape else None,
+                     "traffic_source": TRAFFIC["source"] if same_shape else
+                     f"not measured for this launch shape (profiled: {TRAFFIC['shape']})",
                      "flops_per_launch": stats["flops"], "nn_bytes_per_launch": stats["nn_bytes"],
                      "nn_achieved_gbs": round(nn_gbs, 2), "nn_frac_of_hbm": round(nn_gbs / PEAK_HBM_GBS, 5)},
         "device": dev_name,

@@ -1231,10 +1231,11 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     }
 }
 
-__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, unsigned long long shape) {
+__global__ void reset_kernel(int* ctl, unsigned long long* stats, unsigned* tsync, unsigned long long shape,
+                             unsigned long long fk_tile) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < kCtl) ctl[t] = 0;
-    if (t < kStats) stats[t] = (t == 3) ? shape : 0ull;
+    if (t < kStats) stats[t] = (t == 3) ? shape : (t == 18) ? fk_tile : 0ull;
     const int lines = static_cast<int>(sync_words() / kSyncLine);
     for (int i = t; i < lines; i += gridDim.x * blockDim.x) tsync[static_cast<size_t>(i) * kSyncLine] = 0u;
 }
@@ -1428,7 +1429,8 @@ int planner_impl(hipStream_t stream, int B, const float* starts, const float* go
                                           " bytes of private memory per lane: a cooperative launch of " +
                                           std::to_string(a.slots) + " work-groups would need more than " +
                                           std::to_string(kMaxPassScratch >> 20) + " MiB of scratch at once");
-    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, shape);
+    hipLaunchKernelGGL(reset_kernel, dim3(8), dim3(256), 0, stream, a.ctl, a.stats, a.tsync, shape,
+                       static_cast<unsigned long long>(a.ml.fbp - 1));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
     e = small ? launch_passes<8>(a, nt, lds, stream) : launch_passes<16>(a, nt, lds, stream);

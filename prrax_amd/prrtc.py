@@ -135,7 +135,8 @@ class PlanBuffers:
                "nn_tile_passes": int(raw[17])}
         shape = int(raw[3])
         out["launch"] = {"slots": shape & 0xFFFF, "threads": (shape >> 16) & 0xFFFF,
-                         "pass1_rounds": (shape >> 32) & 0xFFFF, "team_max": (shape >> 48) & 0xFFFF}
+                         "pass1_rounds": (shape >> 32) & 0xFFFF, "team_max": (shape >> 48) & 0xFFFF,
+                         "fk_batch": int(raw[18])}
         out["pass1_ticks"] = {k: int(v) for k, v in zip(self.PHASES, raw[4:10])}
         out["pass2_ticks"] = {k: int(v) for k, v in zip(self.PHASES, raw[10:16])}
         return out
