@@ -13,6 +13,15 @@
 #include <float.h>
 #include <stdint.h>
 
+// Bounds checks and observed queue depths for the CPU build of these functions (a host stand-in
+// for the HIP header defines them there); on the device they expand to nothing.
+#ifndef PRRTC_SIMT_ASSERT
+#define PRRTC_SIMT_ASSERT(cond) ((void)0)
+#endif
+#ifndef PRRTC_SIMT_OBSERVE
+#define PRRTC_SIMT_OBSERVE(slot, value) ((void)0)
+#endif
+
 namespace prrtc {
 
 constexpr int kMaxJoints = 64;
@@ -618,6 +627,10 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     const int ng = M.n_groups, nrs = M.n_spheres;
     const bool mg = M.mode == 1;
     const float cut = mg ? M.margin : 0.0f;
+    PRRTC_SIMT_ASSERT(ng <= M.n_joints && M.n_joints <= kMaxJoints);
+    PRRTC_SIMT_ASSERT(M.goff[ng] <= nrs);
+    PRRTC_SIMT_ASSERT(M.n_wh + M.n_wb + M.n_wc + M.n_ws < (1 << 25));  // entries o << 6 | g
+    PRRTC_SIMT_ASSERT(cull_stride_bytes(nrs, M.n_joints, 0) <= M.cull_stride);
     unsigned lf = 0;  // this lane's flops of fine tests
     auto done = [&](bool v) {
         flops += wave_sum_u32(lf);
@@ -664,6 +677,7 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
         bool h = false;
         for (int e = lane; e < qn && !h; e += kWave) {
             const int o = Q[e] >> 6, g = Q[e] & 63;
+            PRRTC_SIMT_ASSERT(e < kCullQ && o < M.n_wh + M.n_wb + M.n_wc + M.n_ws && g < ng);
             for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
                 if (fine_test_mode(M, mg, o, FP[M.gmem[m]], lf)) {
                     h = true;
@@ -678,8 +692,10 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     };
     auto push = [&](bool dirty, int entry) {
         const unsigned long long mask = __ballot(dirty);
+        PRRTC_SIMT_ASSERT(qn + static_cast<int>(__popcll(mask)) <= kCullQ);
         if (dirty) Q[qn + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
         qn += static_cast<int>(__popcll(mask));
+        PRRTC_SIMT_OBSERVE(0, qn);
     };
     // Pair steps (halfspaces, boxes, capsules): pair p = o * ng + g (obstacle-major), lane l takes
     // p0 + l, and (o, g) advance by kWave pairs per step without a division (ng <= 64).  Sphere
@@ -781,6 +797,10 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
     const int ng = M.n_groups, nrs = M.n_spheres;
     const bool mg = M.mode == 1;
     const float cut = mg ? M.margin : 0.0f;
+    PRRTC_SIMT_ASSERT(nu >= 1 && nu <= kUnitMax && nu <= ts);
+    PRRTC_SIMT_ASSERT(ng <= M.n_joints && ng <= nrs && M.goff[ng] <= nrs);
+    PRRTC_SIMT_ASSERT(M.n_wh + M.n_wb + M.n_wc + M.n_ws < (1 << 21));  // entries o << 10 | u << 6 | g
+    PRRTC_SIMT_ASSERT(cull_stride_bytes(nrs, M.n_joints, kUnitCap) <= M.cull_stride);
     unsigned lf = 0;
     auto done = [&](int v) {
         flops += wave_sum_u32(lf);
@@ -819,6 +839,7 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         if (nl + cnt > kUnitCap) return false;
         if (dirty) LU[nl + static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
         nl += cnt;
+        PRRTC_SIMT_OBSERVE(1, nl);
         return true;
     };
     if (total > 0) {
@@ -877,8 +898,10 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         int entry = 0;
         if (k < nc) {
             const int u = k / nl;
+            PRRTC_SIMT_ASSERT(u < nu && k - u * nl < nl);
             const int en = LU[k - u * nl];
             const int o = en >> 6, g = en & 63;
+            PRRTC_SIMT_ASSERT(o < M.n_wh + M.n_wb + M.n_wc + M.n_ws && g < ng);
             const float4 c = M.gcoarse[g];
             Mat3 R;
             float tx, ty, tz;
@@ -892,9 +915,11 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         const unsigned long long mask = __ballot(dirty);
         if (dirty) Q[static_cast<int>(__popcll(mask & ((1ull << lane) - 1ull)))] = entry;
         const int qn = static_cast<int>(__popcll(mask));  // <= kWave <= kCullQ
+        PRRTC_SIMT_OBSERVE(2, qn);
         wave_sync();
         for (int e = lane; e < qn; e += kWave) {
             const int o = Q[e] >> 10, u = (Q[e] >> 6) & 15, g = Q[e] & 63;
+            PRRTC_SIMT_ASSERT(e < kCullQ && u < nu && g < ng && o < M.n_wh + M.n_wb + M.n_wc + M.n_ws);
             Mat3 R;
             float tx, ty, tz;
             frame_from(T + u, ts, M.gframe[g], R, tx, ty, tz);
