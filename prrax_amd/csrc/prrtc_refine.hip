@@ -16,9 +16,16 @@
 // moving theta_j moves every point x of the subtree by w_j x (x - t_j) + R_j v_j.  Joint
 // gradients then flow to the configuration through the act / mimic maps of the FK.
 //
-// Layout: one wave per configuration, lanes over robot spheres, world spheres broadcast from
-// LDS; per-sphere (joint, g, c x g) go to a per-wave LDS area and are summed per joint in sphere
-// order by one lane per joint, so results are deterministic.  FP32 throughout.
+// Layout: a wave takes a batch of nb configurations (FK one per lane), then the nb * n_spheres
+// (configuration, sphere) items, 64 per wave step, configuration-minor (item k: sphere k / nb,
+// configuration k % nb), so every lane is busy and the items of one configuration sit in the
+// lanes congruent mod nb: a butterfly over lane offsets nb, 2 nb, ..., 32 sums them, in a fixed
+// order (results are deterministic).  Each item loops over the world spheres (LDS broadcast);
+// pairs beyond reach are screened out in squared form.  The adjoint needs, per joint j of the
+// configuration, sum over spheres i on j's subtree of g_i . (w_j x (c_i - t_j) + R_j v_j): each
+// item adds its term for every joint in its frame's ancestor mask, with the joint's world axis,
+// origin and translation direction precomputed per (configuration, joint) in the wave's LDS
+// area, and the per-joint sums go to LDS by the lane that holds the configuration.  FP32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,19 +37,26 @@
 
 namespace prrtc {
 
-// Algorithmic flops (FMA = 2) per robot-world sphere pair: distance test 12 (difference 3,
-// squared norm 5, sqrt, penetration 2, scale) for every pair; for pairs within reach (z >=
-// kFarZ) the softplus 5 more (exp, log1p, max, add, divide) and the gradient 11 more (sigmoid 3,
-// 1/d and scale 2, accumulate 6); per joint of the adjoint 30.
-constexpr unsigned kFlopsPairDist = 12, kFlopsPairCost = 5, kFlopsPairGrad = 11, kFlopsJointAdj = 30;
+// Algorithmic flops (FMA = 2) per robot-world sphere pair: the screen 10 (difference 3, squared
+// norm 5, reach and its square 2) for every pair; for pairs inside the reach the distance 5
+// (square root as d^2 / d, penetration 2, scale 1 ... ); for pairs with z >= kFarZ the softplus 5
+// more (exp, log1p, max, add, scale) and the gradient 11 more (sigmoid 3, 1/d and scale 2,
+// accumulate 6); per (sphere, joint) term of the adjoint 17 (lever arm 3, axis cross 6, plus
+// translation 3, dot 5); per joint 30 for its axis and translation direction.
+constexpr unsigned kFlopsPairScreen = 10, kFlopsPairDist = 5, kFlopsPairCost = 5, kFlopsPairGrad = 11;
+constexpr unsigned kFlopsAdjTerm = 17, kFlopsJointAdj = 30;
+// A pair with z = beta (r_i + r_w - d) < kFarZ contributes below exp(kFarZ) ~ 1e-13 to the cost
+// and its derivative: skipped (z < kFarZ  <=>  d > r_i + r_w - kFarZ / beta).
 constexpr float kFarZ = -30.0f;
+constexpr int kMaxCostBatch = 16;  // configurations per wave batch (nb, a power of two)
 
 struct CostArgs {
     RobotDev robot;
     WorldDev world;
     ModelLayout ml;
-    size_t off_scan, off_acc;
-    int acc_stride;  // floats per wave in the accumulator area
+    size_t off_scan, off_anc, off_jw;
+    int jw_stride;  // float4 per wave in the joint area: 3 * nb * n_joints, then nb * n_joints sums
+    int nb, log_nb;
     const float* configs;
     int n, dim;
     float beta;
@@ -57,10 +71,18 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-__device__ __forceinline__ int wave_sum_int(int v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+// Sum over the lanes congruent to this one mod nb (offsets nb .. 32 of the butterfly).
+__device__ __forceinline__ float class_sum(float v, int nb) {
+    for (int o = nb; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
     return v;
+}
+
+// log(1 + e) for e in [0, 1]: log2 of u = 1 + e rescaled by e / (u - 1), which cancels the
+// rounding of u (accurate to a few ulps, also for tiny e).
+__device__ __forceinline__ float log1p_unit(float e) {
+    const float u = 1.0f + e;
+    if (u == 1.0f) return e;
+    return (__builtin_amdgcn_logf(u) * 0x1.62e430p-1f) * (e * __builtin_amdgcn_rcpf(u - 1.0f));
 }
 
 template <bool GRAD>
@@ -70,153 +92,171 @@ __global__ __launch_bounds__(256) void sphere_cost_kernel(CostArgs a) {
     Model M;
     stage_model(M, lds, a.ml, a.robot, a.world, 0, a.dim, 0, 0.0f, scan);
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const WaveArea W = wave_area(lds, a.ml, M.n_joints);
+    const int nj = M.n_joints, nrs = M.n_spheres, nws = M.n_ws, nb = a.nb, lnb = a.log_nb;
+    // ancestor masks: bit k of anc[j] set iff joint k lies on the path from the root to joint j
+    unsigned long long* anc = reinterpret_cast<unsigned long long*>(lds + a.off_anc);
+    if (GRAD && threadIdx.x == 0) {
+        for (int j = 0; j < nj; ++j) {
+            const int p = M.joints[j].parent;
+            anc[j] = (p >= 0 ? anc[p] : 0ull) | (1ull << j);
+        }
+    }
+    __syncthreads();
+    const WaveArea W = wave_area(lds, a.ml, nj);
     LaneScratch L;
     L.q = W.q + lane;
     L.T = W.T + lane;
     L.nt = kFBP;
-    const int nj = M.n_joints, nrs = M.n_spheres;
-    // accumulator area of this wave: [nrs][7] sphere terms, [nj][6] joint sums, [nj] joint grads
-    float* acc = reinterpret_cast<float*>(lds + a.off_acc) + static_cast<size_t>(wid) * a.acc_stride;
-    float* jsum = acc + 7 * nrs;
-    float* jgrad = jsum + 6 * nj;
-    const float beta = a.beta, inv_beta = 1.0f / a.beta;
+    float4* JW = reinterpret_cast<float4*>(lds + a.off_jw) + static_cast<size_t>(wid) * a.jw_stride;
+    float* JS = reinterpret_cast<float*>(JW + 3 * nb * nj);  // per (configuration, joint) sums
+    const float beta = a.beta, inv_beta = 1.0f / a.beta, reach = -kFarZ / a.beta;
     const int wpb = blockDim.x / kWave;
     const int total_waves = gridDim.x * wpb;
-    unsigned long long flops = 0;
-    const int nb = (a.n + kFB - 1) / kFB;
-    for (int b = blockIdx.x * wpb + wid; b < nb; b += total_waves) {
-        const int i = b * kFB + lane;
-        if (lane < kFB && i < a.n) {
-            for (int d = 0; d < a.dim; ++d) L.q[d * kFBP] = a.configs[static_cast<size_t>(i) * a.dim + d];
+    unsigned long long n_act = 0, n_in = 0, n_near = 0, n_adj = 0;  // this lane's counts
+    const int nbat = (a.n + nb - 1) / nb;
+    const int items = nrs << lnb;
+    for (int b = blockIdx.x * wpb + wid; b < nbat; b += total_waves) {
+        const int c0 = b * nb;
+        const int ncfg = min(nb, a.n - c0);
+        if (lane < ncfg) {
+            for (int d = 0; d < a.dim; ++d) L.q[d * kFBP] = a.configs[static_cast<size_t>(c0 + lane) * a.dim + d];
             if (nj > 0) fk_lane(M, L);
         }
         wave_sync();
-        const int nc = min(kFB, a.n - b * kFB);
-        for (int c = 0; c < nc; ++c) {
-            const int idx = b * kFB + c;
-            const float* T = W.T + c;
-            float cost = 0.0f;
-            int active = 0, near = 0;
-            for (int s = lane; s < nrs; s += kWave) {
-                const int js = M.sphere_js[s];
-                float gx = 0.0f, gy = 0.0f, gz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-                if (js >= 0) {
-                    ++active;
+        if (GRAD) {
+            // (configuration, joint): world axis w (|omega| R a), origin t, translation R v
+            for (int t = lane; t < nb * nj; t += kWave) {
+                const int c = t / nj, j = t - c * nj;
+                float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), o = w, u = w;
+                const JointRec& J = M.joints[j];
+                if (c < ncfg && J.type != kFixed) {
                     Mat3 R;
                     float tx, ty, tz;
-                    frame_from(T, kFBP, js, R, tx, ty, tz);
+                    frame_from(W.T + c, kFBP, j, R, tx, ty, tz);
+                    o = make_float4(tx, ty, tz, 0.0f);
+                    if (J.type == kRot || J.type == kScrew) {
+                        const float3 v = mat_apply(R, 0.0f, 0.0f, 0.0f, J.wn * J.ax, J.wn * J.ay, J.wn * J.az);
+                        w = make_float4(v.x, v.y, v.z, 0.0f);
+                    }
+                    if (J.type == kPrism || J.type == kScrew) {
+                        const float3 v = mat_apply(R, 0.0f, 0.0f, 0.0f, J.vx, J.vy, J.vz);
+                        u = make_float4(v.x, v.y, v.z, 0.0f);
+                    }
+                }
+                JW[3 * t] = w;
+                JW[3 * t + 1] = o;
+                JW[3 * t + 2] = u;
+                JS[t] = 0.0f;
+            }
+            wave_sync();
+        }
+        float cost_c = 0.0f;  // lane c < nb: cost of configuration c0 + c
+        unsigned b_act = 0, b_in = 0, b_near = 0, b_adj = 0;
+        for (int k0 = 0; k0 < items; k0 += kWave) {
+            const int k = k0 + lane;
+            const int s = k >> lnb, c = k & (nb - 1);
+            float cost = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+            unsigned long long am = 0ull;
+            if (k < items && c < ncfg) {
+                const int js = M.sphere_js[s];
+                if (js >= 0) {
+                    ++b_act;
+                    if (GRAD) am = anc[js];
+                    Mat3 R;
+                    float tx, ty, tz;
+                    frame_from(W.T + c, kFBP, js, R, tx, ty, tz);
                     const float4 sp = M.spheres[s];
                     const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
                     cx = p.x; cy = p.y; cz = p.z;
-                    for (int w = 0; w < M.n_ws; ++w) {
+                    const float lim0 = sp.w + reach;
+                    for (int w = 0; w < nws; ++w) {
                         const float4 o = M.wsph[w];
                         const float dx = cx - o.x, dy = cy - o.y, dz = cz - o.z;
-                        const float d = __builtin_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const float lim = lim0 + o.w;
+                        // beyond reach (d > lim, with 1e-5 relative slack for the rounding)
+                        if (!(lim >= 0.0f && d2 <= (lim * lim) * 1.00001f)) continue;
+                        ++b_in;
+                        const float rs = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+                        const float d = d2 * rs;
                         const float z = beta * ((sp.w + o.w) - d);
-                        // A pair with z < kFarZ contributes below exp(kFarZ) ~ 1e-13 (cost and
-                        // derivative alike): skipped.
                         if (z < kFarZ) continue;
-                        ++near;
+                        ++b_near;
                         // softplus(z) = max(z, 0) + log1p(e), sigmoid(z) from the same e = exp(-|z|)
-                        const float e = expf(-fabsf(z));
-                        cost += (fmaxf(z, 0.0f) + log1pf(e)) * inv_beta;
-                        if (GRAD && d > 0.0f) {
-                            const float sg = (z >= 0.0f ? 1.0f : e) / (1.0f + e);  // d softplus_b / dx
-                            const float k = -sg / d;
-                            gx = __builtin_fmaf(k, dx, gx);
-                            gy = __builtin_fmaf(k, dy, gy);
-                            gz = __builtin_fmaf(k, dz, gz);
+                        const float e = __builtin_amdgcn_exp2f(-fabsf(z) * 0x1.715476p+0f);
+                        cost += (fmaxf(z, 0.0f) + log1p_unit(e)) * inv_beta;
+                        if (GRAD) {
+                            const float sg = (z >= 0.0f ? 1.0f : e) * __builtin_amdgcn_rcpf(1.0f + e);
+                            const float kk = -sg * rs;  // 0 when the centres coincide
+                            gx = __builtin_fmaf(kk, dx, gx);
+                            gy = __builtin_fmaf(kk, dy, gy);
+                            gz = __builtin_fmaf(kk, dz, gz);
                         }
                     }
                 }
-                if (GRAD) {
-                    float* e = acc + 7 * s;
-                    e[0] = __int_as_float(js);
-                    e[1] = gx; e[2] = gy; e[3] = gz;
-                    e[4] = __builtin_fmaf(cy, gz, -(cz * gy));
-                    e[5] = __builtin_fmaf(cz, gx, -(cx * gz));
-                    e[6] = __builtin_fmaf(cx, gy, -(cy * gx));
-                }
             }
-            cost = wave_sum(cost);
-            if (lane == 0) a.out_cost[idx] = cost;
-            // flops of this config: FK, sphere transforms, pair terms (+ adjoint)
-            const int act_total = wave_sum_int(active);
-            const int near_total = wave_sum_int(near);
-            if (lane == 0) {
-                flops += static_cast<unsigned long long>(kFlopsJoint) * nj +
-                         static_cast<unsigned long long>(act_total) *
-                             (kFlopsXf + static_cast<unsigned long long>(M.n_ws) * kFlopsPairDist) +
-                         static_cast<unsigned long long>(near_total) * (kFlopsPairCost + (GRAD ? kFlopsPairGrad : 0u));
-                if (GRAD) flops += static_cast<unsigned long long>(kFlopsJointAdj) * nj;
-            }
+            cost_c += class_sum(cost, nb);
             if (GRAD) {
-                wave_sync();
-                // per-joint force and moment sums, in sphere order
-                for (int j = lane; j < nj; j += kWave) {
-                    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
-                    for (int s = 0; s < nrs; ++s) {
-                        const float* e = acc + 7 * s;
-                        if (__float_as_int(e[0]) != j) continue;
-                        f0 += e[1]; f1 += e[2]; f2 += e[3];
-                        m0 += e[4]; m1 += e[5]; m2 += e[6];
+                // this item's term for every joint of its frame's ancestor mask
+                for (int j = 0; j < nj; ++j) {
+                    if (M.joints[j].type == kFixed) continue;  // wave-uniform
+                    float v = 0.0f;
+                    if ((am >> j) & 1ull) {
+                        ++b_adj;
+                        const float4* e = JW + 3 * (c * nj + j);
+                        const float4 w = e[0], o = e[1], u = e[2];
+                        const float lx = cx - o.x, ly = cy - o.y, lz = cz - o.z;
+                        const float vx = __builtin_fmaf(w.y, lz, -(w.z * ly)) + u.x;
+                        const float vy = __builtin_fmaf(w.z, lx, -(w.x * lz)) + u.y;
+                        const float vz = __builtin_fmaf(w.x, ly, -(w.y * lx)) + u.z;
+                        v = __builtin_fmaf(gz, vz, __builtin_fmaf(gy, vy, gx * vx));
                     }
-                    float* o = jsum + 6 * j;
-                    o[0] = f0; o[1] = f1; o[2] = f2; o[3] = m0; o[4] = m1; o[5] = m2;
+                    v = class_sum(v, nb);
+                    if (lane < nb) JS[lane * nj + j] += v;
                 }
-                wave_sync();
-                // subtree sums: children follow their parents in joint order
-                if (lane == 0) {
-                    for (int j = nj - 1; j > 0; --j) {
-                        const int p = M.joints[j].parent;
-                        if (p < 0) continue;
-                        for (int k = 0; k < 6; ++k) jsum[6 * p + k] += jsum[6 * j + k];
-                    }
-                }
-                wave_sync();
-                for (int j = lane; j < nj; j += kWave) {
+            }
+        }
+        n_act += b_act;
+        n_in += b_in;
+        n_near += b_near;
+        n_adj += b_adj;
+        if (lane < ncfg) a.out_cost[c0 + lane] = cost_c;
+        if (GRAD) {
+            wave_sync();
+            // joint sums to the configuration through the act / mimic maps, lanes over (c, d)
+            for (int t = lane; t < nb * a.dim; t += kWave) {
+                const int c = t / a.dim, d = t - c * a.dim;
+                if (c >= ncfg) continue;
+                float gd = 0.0f;
+                for (int j = 0; j < nj; ++j) {
                     const JointRec& J = M.joints[j];
-                    float g = 0.0f;
-                    if (J.type != kFixed) {
-                        Mat3 R;
-                        float tx, ty, tz;
-                        frame_from(T, kFBP, j, R, tx, ty, tz);
-                        const float* f = jsum + 6 * j;
-                        if (J.type == kRot || J.type == kScrew) {
-                            // world axis times |omega|; moment about the joint origin
-                            const float3 w = mat_apply(R, 0.0f, 0.0f, 0.0f, J.wn * J.ax, J.wn * J.ay, J.wn * J.az);
-                            const float mx = f[3] - __builtin_fmaf(ty, f[2], -(tz * f[1]));
-                            const float my = f[4] - __builtin_fmaf(tz, f[0], -(tx * f[2]));
-                            const float mz = f[5] - __builtin_fmaf(tx, f[1], -(ty * f[0]));
-                            g = __builtin_fmaf(w.z, mz, __builtin_fmaf(w.y, my, w.x * mx));
-                        }
-                        if (J.type == kPrism || J.type == kScrew) {
-                            const float3 v = mat_apply(R, 0.0f, 0.0f, 0.0f, J.vx, J.vy, J.vz);
-                            g = __builtin_fmaf(v.z, f[2], __builtin_fmaf(v.y, f[1], __builtin_fmaf(v.x, f[0], g)));
-                        }
+                    if (J.mimic >= 0) {
+                        if (J.mimic == d) gd = __builtin_fmaf(J.mul, JS[c * nj + j], gd);
+                    } else if (J.act == d) {
+                        gd += JS[c * nj + j];
                     }
-                    jgrad[j] = g;
                 }
-                wave_sync();
-                for (int d = lane; d < a.dim; d += kWave) {
-                    float gd = 0.0f;
-                    for (int j = 0; j < nj; ++j) {
-                        const JointRec& J = M.joints[j];
-                        if (J.mimic >= 0) {
-                            if (J.mimic == d) gd = __builtin_fmaf(J.mul, jgrad[j], gd);
-                        } else if (J.act == d) {
-                            gd += jgrad[j];
-                        }
-                    }
-                    a.out_grad[static_cast<size_t>(idx) * a.dim + d] = gd;
-                }
-                wave_sync();
+                a.out_grad[static_cast<size_t>(c0 + c) * a.dim + d] = gd;
             }
         }
         wave_sync();
     }
-    if (lane == 0 && a.flops && flops) global_add(a.flops, flops);
+    if (a.flops) {
+        // FK and adjoint set-up per configuration, then the per-item and per-pair counts
+        const unsigned long long mine =
+            n_act * (kFlopsXf + static_cast<unsigned long long>(nws) * kFlopsPairScreen) + n_in * kFlopsPairDist +
+            n_near * (kFlopsPairCost + (GRAD ? kFlopsPairGrad : 0u)) + n_adj * kFlopsAdjTerm;
+        unsigned long long tot = mine;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
+        if (lane == 0) {
+            unsigned long long cfgs = 0;
+            for (int b = blockIdx.x * wpb + wid; b < nbat; b += total_waves) cfgs += min(nb, a.n - b * nb);
+            tot += cfgs * (static_cast<unsigned long long>(kFlopsJoint) * nj +
+                           (GRAD ? static_cast<unsigned long long>(kFlopsJointAdj) * nj : 0ull));
+            if (tot) global_add(a.flops, tot);
+        }
+    }
 }
 
 // Jacobi step of every interior waypoint: candidate = clamp(q - step * (dC/dq + dS/dq)), with
@@ -350,16 +390,26 @@ int plan_cost(const prrtc_robot* robot, const float* world_spheres, int nws, int
     w.n_world_spheres = nws;
     CostArgs& a = cp.a;
     std::memset(&a, 0, sizeof(a));
-    int nt = 256;
     const int nj = robot->n_joints, nrs = robot->n_robot_spheres;
-    a.acc_stride = static_cast<int>((7 * nrs + 7 * nj + 3) & ~3);
+    // Configurations per wave batch: the largest power of two <= 16 that still gives every SIMD
+    // 8 waves of batches (small n favours more, smaller batches), at least 4 (FK runs on nb
+    // lanes, so smaller batches repeat it for too few configurations).
+    int nb = kMaxCostBatch;
+    while (nb > 4 && (static_cast<long long>(n) + nb - 1) / nb < 32ll * di.cus) nb /= 2;
+    int nt = 256;
     for (;;) {
         a.ml = model_layout(nj, nrs, 0, nws, 0, 0, 0, nt / kWave, kFBP);
-        a.off_acc = align16(a.ml.end);
-        a.off_scan = align16(a.off_acc + sizeof(float) * a.acc_stride * (nt / kWave));
+        a.nb = nb;
+        a.log_nb = __builtin_ctz(static_cast<unsigned>(nb));
+        a.jw_stride = 3 * nb * nj + (nb * nj + 3) / 4;
+        a.off_anc = align16(a.ml.end);
+        a.off_jw = align16(a.off_anc + sizeof(unsigned long long) * nj);
+        a.off_scan = align16(a.off_jw + sizeof(float4) * a.jw_stride * (nt / kWave));
         cp.lds = align16(a.off_scan + sizeof(int) * 20);
-        if (cp.lds <= static_cast<size_t>(di.max_lds) || nt <= kWave) break;
-        nt -= kWave;
+        if (cp.lds <= static_cast<size_t>(di.max_lds)) break;
+        if (nb > 4) nb /= 2;
+        else if (nt > kWave) nt -= kWave;
+        else break;
     }
     if (cp.lds > static_cast<size_t>(di.max_lds))
         return fail_with(PRRTC_E_LDS, "robot + world spheres exceed the work-group LDS");
@@ -369,7 +419,7 @@ int plan_cost(const prrtc_robot* robot, const float* world_spheres, int nws, int
     a.dim = dim;
     a.beta = beta;
     cp.nt = nt;
-    const int batches = (n + kFB - 1) / kFB;
+    const int batches = (n + nb - 1) / nb;
     cp.grid = std::max(1, std::min((batches + nt / kWave - 1) / (nt / kWave), 8 * di.cus));
     return PRRTC_OK;
 }
