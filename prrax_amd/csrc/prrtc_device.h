@@ -248,7 +248,7 @@ __device__ __forceinline__ bool near_halfspace(float sx, float sy, float sz, flo
     return (__builtin_fmaf(h.z, sz, __builtin_fmaf(h.y, sy, h.x * sx)) - h.w) - sr < margin;
 }
 
-// Signed distances (the minimum-distance value of the flat check).  The square root is the
+// Signed distances (the minimum-distance value of the flat sweep).  The square root is the
 // hardware's (v_sqrt_f32, within 1 ulp), not the correctly rounded one: no verdict depends on
 // these values.
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -258,19 +258,17 @@ __device__ __forceinline__ float dist_sphere(float sx, float sy, float sz, float
     return fast_sqrt(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) - (sr + o.w);
 }
 
-__device__ __forceinline__ float dist_capsule(float sx, float sy, float sz, float sr, const CapsuleL& c) {
-    return fast_sqrt(capsule_d2(sx, sy, sz, c)) - (sr + c.r);
+// Signed distances of a point (a robot sphere's centre) to the obstacle surfaces; the flat sweep
+// subtracts the robot sphere's radius once from their minimum.
+__device__ __forceinline__ float gap_sphere(float sx, float sy, float sz, const float4 o) {
+    const float dx = sx - o.x, dy = sy - o.y, dz = sz - o.z;
+    return fast_sqrt(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx))) - o.w;
 }
 
-__device__ __forceinline__ float dist_box(float sx, float sy, float sz, float sr, const BoxL& b) {
+__device__ __forceinline__ float gap_box(float sx, float sy, float sz, const BoxL& b) {
     float qmax;
     const float outside = fast_sqrt(box_d2(sx, sy, sz, b, qmax));
-    const float inside = fminf(qmax, 0.0f);
-    return (outside + inside) - sr;
-}
-
-__device__ __forceinline__ float dist_halfspace(float sx, float sy, float sz, float sr, const float4 h) {
-    return (__builtin_fmaf(h.z, sz, __builtin_fmaf(h.y, sy, h.x * sx)) - h.w) - sr;
+    return outside + fminf(qmax, 0.0f);
 }
 
 // ---------------------------------------------------------------------------------- FK
@@ -364,9 +362,9 @@ __device__ __forceinline__ void fk_lane(const Model& M, const LaneScratch& L) {
 
 // ---------------------------------------------------------------------------------- wave checks
 // One wave checks one configuration: lane l takes robot spheres l, l+64, ...; every lane walks the
-// same obstacle list (LDS broadcast), and the wave leaves as soon as any lane reports a contact
-// (ballot), so the exit is wave-uniform.  The verdicts are an OR (binary) or a min (margin) over
-// all tests, so the order of the tests does not change the result.
+// same obstacle list (LDS broadcast), and a verdict leaves as soon as any lane reports a contact
+// (ballot), so the exit is wave-uniform.  The verdicts are an OR of pair tests and the distance
+// a min over pairs, so the order of the tests does not change the result.
 //
 // The configuration's joint frames come from a per-wave FK area T (element c of slot k at
 // T[k * ts]); fk_lane fills it for a batch of configurations, one per lane.
@@ -393,9 +391,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <bool MARGIN>
-__device__ __forceinline__ float wave_check(const Model& M, const float* T, int ts, unsigned long long& flops) {
+// Minimum signed distance of one configuration over every robot-obstacle sphere pair and self
+// pair: the flat full sweep (no culling, no early exit).  Lane l takes robot spheres l, l + 64,
+// ...; per sphere it keeps the minimum of its centre's distances to the obstacle surfaces and
+// subtracts its radius once (the value differs from a per-pair "distance - radii" only by the
+// rounding of that subtraction).  World primitives go in unrolled blocks (8 spheres, 4 capsules,
+// 4 boxes) with no per-element guard, so each block's LDS broadcasts are issued together, and
+// the minimum of a block is a tree of three-operand minima.
+__device__ __forceinline__ float wave_min_distance(const Model& M, const float* T, int ts, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
+    const int nws = M.n_ws, nwc = M.n_wc, nwb = M.n_wb, nwh = M.n_wh;
+    const unsigned long long per_sphere = static_cast<unsigned long long>(nws) * kFlopsSS +
+                                          static_cast<unsigned long long>(nwc) * kFlopsSC +
+                                          static_cast<unsigned long long>(nwb) * kFlopsSB +
+                                          static_cast<unsigned long long>(nwh) * kFlopsSH;
     float best = FLT_MAX;
     for (int s0 = 0; s0 < M.n_spheres; s0 += kWave) {
         const int s = s0 + lane;
@@ -413,61 +422,40 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
             const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
             px = p.x; py = p.y; pz = p.z;
         }
-        flops += na * kFlopsXf;
-        float cb = FLT_MAX;  // this chunk's minimum (margin mode)
-        for (int i = 0; i < M.n_ws; i += 8) {
-            const int ne = min(8, M.n_ws - i);
-            bool h = false;
+        flops += na * (kFlopsXf + per_sphere);
+        float cb = FLT_MAX;
+        int i = 0;
+        for (; i + 8 <= nws; i += 8) {
+            float g[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k < ne) {
-                    if (MARGIN) cb = fminf(cb, dist_sphere(px, py, pz, sp.w, M.wsph[i + k]));
-                    else h |= hit_sphere(px, py, pz, sp.w, M.wsph[i + k]);
-                }
-            }
-            flops += static_cast<unsigned long long>(na) * ne * kFlopsSS;
-            if (!MARGIN && wave_any(act && h)) return 1.0f;
+            for (int k = 0; k < 8; ++k) g[k] = gap_sphere(px, py, pz, M.wsph[i + k]);
+            cb = fminf(cb, fminf(fminf(fminf(g[0], g[1]), fminf(g[2], g[3])), fminf(fminf(g[4], g[5]), fminf(g[6], g[7]))));
         }
-        for (int i = 0; i < M.n_wc; i += 4) {
-            const int ne = min(4, M.n_wc - i);
-            bool h = false;
+        for (; i < nws; ++i) cb = fminf(cb, gap_sphere(px, py, pz, M.wsph[i]));
+        i = 0;
+        for (; i + 4 <= nwc; i += 4) {
+            float g[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (k < ne) {
-                    if (MARGIN) cb = fminf(cb, dist_capsule(px, py, pz, sp.w, M.wcap[i + k]));
-                    else h |= hit_capsule(px, py, pz, sp.w, M.wcap[i + k]);
-                }
-            }
-            flops += static_cast<unsigned long long>(na) * ne * kFlopsSC;
-            if (!MARGIN && wave_any(act && h)) return 1.0f;
+            for (int k = 0; k < 4; ++k) g[k] = fast_sqrt(capsule_d2(px, py, pz, M.wcap[i + k])) - M.wcap[i + k].r;
+            cb = fminf(cb, fminf(fminf(g[0], g[1]), fminf(g[2], g[3])));
         }
-        for (int i = 0; i < M.n_wb; i += 4) {
-            const int ne = min(4, M.n_wb - i);
-            bool h = false;
+        for (; i < nwc; ++i) cb = fminf(cb, fast_sqrt(capsule_d2(px, py, pz, M.wcap[i])) - M.wcap[i].r);
+        i = 0;
+        for (; i + 4 <= nwb; i += 4) {
+            float g[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (k < ne) {
-                    if (MARGIN) cb = fminf(cb, dist_box(px, py, pz, sp.w, M.wbox[i + k]));
-                    else h |= hit_box(px, py, pz, sp.w, M.wbox[i + k]);
-                }
-            }
-            flops += static_cast<unsigned long long>(na) * ne * kFlopsSB;
-            if (!MARGIN && wave_any(act && h)) return 1.0f;
+            for (int k = 0; k < 4; ++k) g[k] = gap_box(px, py, pz, M.wbox[i + k]);
+            cb = fminf(cb, fminf(fminf(g[0], g[1]), fminf(g[2], g[3])));
         }
-        {
-            bool h = false;
-            for (int k = 0; k < M.n_wh; ++k) {
-                if (MARGIN) cb = fminf(cb, dist_halfspace(px, py, pz, sp.w, M.whs[k]));
-                else h |= hit_halfspace(px, py, pz, sp.w, M.whs[k]);
-            }
-            flops += static_cast<unsigned long long>(na) * M.n_wh * kFlopsSH;
-            if (!MARGIN && wave_any(act && h)) return 1.0f;
+        for (; i < nwb; ++i) cb = fminf(cb, gap_box(px, py, pz, M.wbox[i]));
+        for (int k = 0; k < nwh; ++k) {
+            const float4 h = M.whs[k];
+            cb = fminf(cb, __builtin_fmaf(h.z, pz, __builtin_fmaf(h.y, py, h.x * px)) - h.w);
         }
-        if (MARGIN && act) best = fminf(best, cb);
+        if (act) best = fminf(best, cb - sp.w);
     }
     for (int p0 = 0; p0 < M.n_pairs; p0 += kWave) {
         const int k = p0 + lane;
-        bool h = false;
         if (k < M.n_pairs) {
             const int pi = static_cast<int>(M.pairs[k] & 0xffffu), pj = static_cast<int>(M.pairs[k] >> 16);
             const float4 a = M.spheres[pi];
@@ -478,20 +466,18 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
             frame_from(T, ts, M.sphere_js[pj], Rj, jx, jy, jz);
             const float3 pa = mat_apply(Ri, ix, iy, iz, a.x, a.y, a.z);
             const float3 pb = mat_apply(Rj, jx, jy, jz, b.x, b.y, b.z);
-            if (MARGIN) best = fminf(best, dist_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w)));
-            else h = hit_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w));
+            best = fminf(best, dist_sphere(pa.x, pa.y, pa.z, a.w, make_float4(pb.x, pb.y, pb.z, b.w)));
         }
         flops += static_cast<unsigned long long>(min(kWave, M.n_pairs - p0)) * kFlopsPair;
-        if (!MARGIN && wave_any(h)) return 1.0f;
     }
-    if (!MARGIN) return 0.0f;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) best = fminf(best, __shfl_xor(best, o, kWave));
     return best;
 }
 
 // ---------------------------------------------------------------------------------- culled checks
-// The same verdicts as wave_check, with less work: each frame's robot spheres are enclosed in a
+// The verdict of testing every robot sphere against every obstacle and every self pair (the
+// reference's exhaustive loop), with less work: each frame's robot spheres are enclosed in a
 // bounding sphere (centre c, radius R = max |p - c| + |r| over its spheres), and a fine test of
 // obstacle o against the spheres of group g runs only if the bounding sphere does not prove them
 // all clear.  Proof: every fine test is a 1-Lipschitz distance of the sphere centre (point,
@@ -501,7 +487,7 @@ __device__ __forceinline__ float wave_check(const Model& M, const float* T, int 
 // (d <= |r + r_o|) and, in margin mode (cut = margin), any distance below the margin.  kCullPad
 // (1 mm) covers the FP32 rounding of both transforms; boxes whose axes are not orthonormal and
 // halfspaces whose normal is not unit are never culled.  Every fine test that does run is the
-// flat check's own test on the same FP32 values, so the verdict is identical bit for bit.
+// exhaustive loop's own test on the same FP32 values, so the verdict is identical bit for bit.
 constexpr float kCullPad = 1e-3f;
 constexpr int kCullQ = 2 * kWave;     // refinement queue (obstacle * 64 + group) per wave
 constexpr int kUnitCap = 2 * kWave;   // candidate pairs of a sub-unit (wave_unit_check)
@@ -548,7 +534,7 @@ __device__ __forceinline__ bool coarse_clear(const Model& M, int o, const float4
 }
 
 // Fine test of robot sphere q (world x, y, z, r) against obstacle o: hit (binary) or distance
-// below the margin (MARGIN), exactly as the flat check computes it.  The checks below take the
+// below the margin (MARGIN), as the exhaustive loop computes it.  The checks below take the
 // mode at run time (M.mode == 1: margin; it is wave-uniform) and branch to one of the two
 // instances at each fine-test site, so each check exists once per kernel.
 template <bool MARGIN>
@@ -889,7 +875,7 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
     // ---- world: every (configuration, candidate) pair at once, lanes over pairs.  The coarse
     // test uses the configuration's own group sphere (wave_culled's); the fine tests of a queued
     // (configuration, obstacle, group) triple transform the group's member spheres with their
-    // frame exactly as wave_check does.  hm: bit u set if configuration u hits the world.
+    // frame as wave_culled does.  hm: bit u set if configuration u hits the world.
     unsigned hm = 0;
     const int nc = nu * nl;
     for (int k0 = 0; k0 < nc; k0 += kWave) {
@@ -980,7 +966,7 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
 }
 
 // The planner's verdict for one configuration (prrtc_planner.cu:501-531), wave-uniform; the
-// culled check gives the flat check's verdict (see wave_culled).
+// culled check gives the exhaustive loop's verdict (see wave_culled).
 __device__ __forceinline__ bool wave_in_collision(const Model& M, const float* T, int ts, unsigned long long& flops) {
     if (!M.enabled) return false;
     flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(1024) void check_kernel(CheckArgs a) {
             const int idx = b * kFB + c;
             if (a.out_min) {
                 flops += static_cast<unsigned long long>(kFlopsJoint) * M.n_joints;
-                const float md = wave_check<true>(M, W.T + c, kFBP, flops);
+                const float md = wave_min_distance(M, W.T + c, kFBP, flops);
                 if (lane == 0) a.out_min[idx] = md;
                 if (a.out_hit) {
                     const bool h = wave_in_collision(M, W.T + c, kFBP, flops);

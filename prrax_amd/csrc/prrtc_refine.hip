@@ -77,12 +77,13 @@ __device__ __forceinline__ float class_sum(float v, int nb) {
     return v;
 }
 
-// log(1 + e) for e in [0, 1]: log2 of u = 1 + e rescaled by e / (u - 1), which cancels the
-// rounding of u (accurate to a few ulps, also for tiny e).
+// log(1 + e) for e in [0, 1]: below 2^-6 the series e - e^2/2 + e^3/3 - e^4/4 (truncation below
+// e^5/5, relative 2e-9), above it log2(1 + e) ln 2 (the rounding of 1 + e is then below 4e-6
+// of the result); both are computed and one is selected, so the lanes do not diverge.
 __device__ __forceinline__ float log1p_unit(float e) {
-    const float u = 1.0f + e;
-    if (u == 1.0f) return e;
-    return (__builtin_amdgcn_logf(u) * 0x1.62e430p-1f) * (e * __builtin_amdgcn_rcpf(u - 1.0f));
+    const float ser = e * __builtin_fmaf(e, __builtin_fmaf(e, __builtin_fmaf(e, -0.25f, 0x1.555556p-2f), -0.5f), 1.0f);
+    const float lg = __builtin_amdgcn_logf(1.0f + e) * 0x1.62e430p-1f;
+    return e < 0x1p-6f ? ser : lg;
 }
 
 template <bool GRAD>
@@ -173,9 +174,10 @@ __global__ __launch_bounds__(256) void sphere_cost_kernel(CostArgs a) {
                         const float4 o = M.wsph[w];
                         const float dx = cx - o.x, dy = cy - o.y, dz = cz - o.z;
                         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        const float lim = lim0 + o.w;
-                        // beyond reach (d > lim, with 1e-5 relative slack for the rounding)
-                        if (!(lim >= 0.0f && d2 <= (lim * lim) * 1.00001f)) continue;
+                        // beyond reach: d > lim (1e-5 relative slack for the rounding; a negative
+                        // reach lets only coincident centres through, which the z test rejects)
+                        const float lim = fmaxf(lim0 + o.w, 0.0f);
+                        if (!(d2 <= (lim * lim) * 1.00001f)) continue;
                         ++b_in;
                         const float rs = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
                         const float d = d2 * rs;
@@ -392,10 +394,11 @@ int plan_cost(const prrtc_robot* robot, const float* world_spheres, int nws, int
     std::memset(&a, 0, sizeof(a));
     const int nj = robot->n_joints, nrs = robot->n_robot_spheres;
     // Configurations per wave batch: the largest power of two <= 16 that still gives every SIMD
-    // 8 waves of batches (small n favours more, smaller batches), at least 4 (FK runs on nb
-    // lanes, so smaller batches repeat it for too few configurations).
+    // 4 batches, at least 4 (FK runs on nb lanes, so smaller batches repeat it for too few
+    // configurations).  32768 configurations of a 7-DOF arm: nb 8 takes 0.159 ms on MI355X,
+    // nb 4 0.204 ms (FK repeated for fewer configurations), nb 16 0.231 ms (too few waves).
     int nb = kMaxCostBatch;
-    while (nb > 4 && (static_cast<long long>(n) + nb - 1) / nb < 32ll * di.cus) nb /= 2;
+    while (nb > 4 && (static_cast<long long>(n) + nb - 1) / nb < 16ll * di.cus) nb /= 2;
     int nt = 256;
     for (;;) {
         a.ml = model_layout(nj, nrs, 0, nws, 0, 0, 0, nt / kWave, kFBP);

@@ -1,6 +1,6 @@
 // CPU build of the planner's wave-level collision checks (prrtc_device.h / prrtc_common.h), for
 // bounds checking under AddressSanitizer.  One work-group of one wave: 64 threads run the device
-// code of stage_model, fk_lane, wave_check (flat), wave_in_collision (culled) and
+// code of stage_model, fk_lane, wave_min_distance (flat sweep), wave_in_collision (culled) and
 // wave_unit_in_collision (sub-units), exactly as the kernels call them, over the configurations
 // of an input file.  The LDS of the work-group is one heap block of the model layout's size.
 //
@@ -13,7 +13,7 @@
 // f32, self_pairs i32, params i32 [dim, mode, n_configs, n_units], margin f32 [1], configs f32
 // [n_configs * dim], units i32 [n_units * 2] (first configuration, count; a unit lies inside one
 // batch of 16 consecutive configurations).
-// Output: hit_flat u8[n], hit_culled u8[n], min_distance f32[n], unit verdict i32[n_units],
+// Output: hit_culled u8[n], min_distance f32[n], unit verdict i32[n_units],
 // observed maxima i64[4] (culled queue depth, unit candidate list length, unit refinement queue
 // depth, groups).
 #include <hip/hip_runtime.h>
@@ -119,7 +119,7 @@ int main(int argc, char** argv) {
     const ModelLayout ml = model_layout(nj, nrs, w.n_sp, w.n_ws, w.n_wc, w.n_wb, w.n_wh, 1, kFBP, kUnitCap);
     std::vector<char> lds(ml.end);  // the work-group's dynamic LDS, exactly sized
     std::vector<int> scan(17);
-    std::vector<uint8_t> hit_flat(n), hit_culled(n);
+    std::vector<uint8_t> hit_culled(n);
     std::vector<float> min_d(n);
     std::vector<int> unit_v(nun);
     blockDim.x = simt::kLanes;
@@ -147,12 +147,10 @@ int main(int argc, char** argv) {
             wave_sync();
             const int nc = min(kFB, n - b0);
             for (int c = 0; c < nc; ++c) {
-                const float md = wave_check<true>(M, W.T + c, kFBP, flops);
-                const float hf = M.enabled ? wave_check<false>(M, W.T + c, kFBP, flops) : 0.0f;
+                const float md = wave_min_distance(M, W.T + c, kFBP, flops);
                 const bool hc = wave_in_collision(M, W.T + c, kFBP, flops);
                 if (lane == 0) {
                     min_d[b0 + c] = md;
-                    hit_flat[b0 + c] = hf != 0.0f ? 1 : 0;
                     hit_culled[b0 + c] = hc ? 1 : 0;
                 }
             }
@@ -175,7 +173,6 @@ int main(int argc, char** argv) {
     for (int l = 0; l < simt::kLanes; ++l) lanes.emplace_back(body, l);
     for (auto& t : lanes) t.join();
     std::ofstream out(argv[2], std::ios::binary);
-    out.write(reinterpret_cast<const char*>(hit_flat.data()), n);
     out.write(reinterpret_cast<const char*>(hit_culled.data()), n);
     out.write(reinterpret_cast<const char*>(min_d.data()), static_cast<std::streamsize>(4 * n));
     out.write(reinterpret_cast<const char*>(unit_v.data()), static_cast<std::streamsize>(4 * nun));

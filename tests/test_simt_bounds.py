@@ -1,7 +1,7 @@
 """CPU build of the planner's wave-level collision checks, run under AddressSanitizer with bounds
 assertions on their LDS queues and packed entries (tests/simt).
 
-The device functions stage_model, fk_lane, wave_check, wave_in_collision and
+The device functions stage_model, fk_lane, wave_min_distance, wave_in_collision and
 wave_unit_in_collision are compiled with g++ against a host stand-in for the HIP header that runs
 one wave as 64 threads.  Every refinement-queue push (at most 2 * 64 entries), candidate-list push
 (at most 2 * 64), packed entry (obstacle, configuration, group) and decoded index is asserted in
@@ -68,12 +68,11 @@ def run_simt(exe: Path, tmp: Path, cc: dict, dim: int, q: np.ndarray, units: np.
     raw = out.read_bytes()
     nu = units.shape[0]
     o = 0
-    hit_flat = np.frombuffer(raw, np.uint8, n, o); o += n
     hit_culled = np.frombuffer(raw, np.uint8, n, o); o += n
     md = np.frombuffer(raw, np.float32, n, o); o += 4 * n
     uv = np.frombuffer(raw, np.int32, nu, o); o += 4 * nu
     obs = np.frombuffer(raw, np.int64, 4, o)
-    return {"hit_flat": hit_flat.astype(bool), "hit_culled": hit_culled.astype(bool), "min": md, "unit": uv,
+    return {"hit_culled": hit_culled.astype(bool), "min": md, "unit": uv,
             "max_queue": int(obs[0]), "max_candidates": int(obs[1]), "max_unit_queue": int(obs[2]),
             "groups": int(obs[3])}
 
@@ -106,9 +105,6 @@ def check_case(exe, tmp, cc, dim, q, mode, margin, expect_hits=(0.02, 0.98)):
     ctx = O.Context(cc, dim, collision_mode=mode, collision_margin=margin)
     want = O.in_collision(ctx, q)
     np.testing.assert_array_equal(res["hit_culled"], want)
-    # the flat check is binary: in margin mode it is the contact verdict of the margin-free context
-    flat_want = want if mode == 0 else O.in_collision(O.Context(cc, dim, collision_mode=0), q)
-    np.testing.assert_array_equal(res["hit_flat"], flat_want)
     md = O.min_distance(O.Context(cc, dim, collision_mode=1, collision_margin=0.0), q)
     np.testing.assert_allclose(res["min"], md, rtol=0, atol=1e-6)
     lo_h, hi_h = expect_hits
