@@ -86,10 +86,14 @@ def tuning_sample(n_batch: int, tail: Sequence[int], n: int = 256, n_tail: int =
 def autotune(starts, goals, collision_context, *, max_nodes_per_query=None,
              samples: Sequence[int] = (128, 256, 512), threads: Sequence[int] = (256, 512),
              team_max: Sequence[int] = (0, 128), fk_batch: Sequence[int] = (16, 8), n_sample: int = 256,
-             n_tail: int = 16, probe_rounds: int = 24, repeats: int = 1, log=None, **plan_kw) -> TuneResult:
+             n_tail: int = 16, probe_rounds: int = 24, repeats: int = 1, min_gain: float = 0.03, log=None,
+             **plan_kw) -> TuneResult:
     """Times candidates on a tail-seeded sample of the batch and returns the fastest whose solve
-    rate is at least S = 128's.  ``plan_kw`` are the planner keywords of the real run
-    (max_iterations, step_size, limits, ...); ``collision_context`` is one dict or one per query."""
+    rate is at least S = 128's.  A candidate replaces the defaults of its stage (S = 128; then the
+    library's launch shape at the chosen S) only if it is faster by more than ``min_gain`` on
+    the sample: smaller differences are within what a sample predicts of the whole batch.
+    ``plan_kw`` are the planner keywords of the real run (max_iterations, step_size, limits,
+    ...); ``collision_context`` is one dict or one per query."""
     B = len(starts)
     shared = None if isinstance(collision_context, (list, tuple)) else collision_context
     ctxs = collision_context if shared is None else None
@@ -133,7 +137,10 @@ def autotune(starts, goals, collision_context, *, max_nodes_per_query=None,
     base = {S: run(S, 0, 0, 0) for S in samples}
     floor = base[DEFAULT_SAMPLES]["solve_rate"] if DEFAULT_SAMPLES in base else 0.0
     ok = [r for r in base.values() if r["solve_rate"] >= floor] or list(base.values())
-    S_best = max(ok, key=lambda r: r["plans_per_s"])["num_new_samples"]
+    top = max(ok, key=lambda r: r["plans_per_s"])
+    dflt = base.get(DEFAULT_SAMPLES, top)
+    S_best = top["num_new_samples"] if top["plans_per_s"] > (1.0 + min_gain) * dflt["plans_per_s"] \
+        else dflt["num_new_samples"]
     # stage 2: the launch shape at that S (results do not depend on it)
     for nt in threads:
         for tm in team_max:
@@ -141,6 +148,9 @@ def autotune(starts, goals, collision_context, *, max_nodes_per_query=None,
                 run(S_best, nt, tm, fb)
     cands = [r for r in table if r["num_new_samples"] == S_best]
     best = max(cands, key=lambda r: r["plans_per_s"])
+    shape0 = base[S_best]  # the library's launch defaults at S_best
+    if best["plans_per_s"] <= (1.0 + min_gain) * shape0["plans_per_s"]:
+        best = shape0
     return TuneResult(best["num_new_samples"],
                       {"threads": best["threads"], "team_max": best["team_max"], "fk_batch": best["fk_batch"]},
                       best["plans_per_s"], best["solve_rate"], table, idx, [int(t) for t in tail])
