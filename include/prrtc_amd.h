@@ -253,6 +253,43 @@ int prrtc_refine_paths(prrtc_stream_t stream, float* paths, int32_t n_paths, int
                        float smooth_weight, float step, int32_t iterations, float* out_cost, void* workspace,
                        size_t workspace_bytes, uint64_t* work_flops);
 
+/* Environment point cloud for edge validation.  The reference's point-cloud planner validates
+ * its edges against a cloud on pyroffi's OptiX checker, each point a sphere of radius r_env
+ * (prrtc_robogpu.py:103-298, README.md:107-148); that checker is absent from the reference tree,
+ * so this one is defined here: a robot sphere (c, r) hits point p iff |c - p| <= r + r_env.
+ * The cloud lives in a uniform grid of cubic cells of edge `cell` over [lo, hi] (points outside
+ * go to the nearest border cell), points sorted by cell; prrtc_cloud_build fills this struct,
+ * whose pointers are DEVICE memory inside the caller's buffer. */
+typedef struct {
+    const float* points;       /* [n_points, 4] (x, y, z, 0), sorted by cell */
+    const int32_t* cell_start; /* [nx * ny * nz + 1]: cell (x, y, z) = (z ny + y) nx + x holds
+                                  points[cell_start[c] .. cell_start[c + 1]) */
+    int32_t n_points;
+    int32_t nx, ny, nz;
+    float lo[3];
+    float cell;
+    float r_env;
+} prrtc_cloud;
+
+/* Bytes of device memory prrtc_cloud_build needs (0 for invalid arguments).  lo, hi: HOST
+ * float[3], the grid's bounds; the grid may hold at most 2^24 cells. */
+size_t prrtc_cloud_bytes(int32_t n_points, const float* lo, const float* hi, float cell);
+
+/* Builds the grid of points [n_points, 3] (device) into `memory` on `stream` (count, scan and
+ * scatter kernels, no host synchronisation) and fills *out.  No reference counterpart (it stands
+ * where pyroffi's OptiX BVH build stood). */
+int prrtc_cloud_build(prrtc_stream_t stream, const float* points, int32_t n_points, const float* lo, const float* hi,
+                      float cell, float r_env, void* memory, size_t memory_bytes, prrtc_cloud* out);
+
+/* Edge validation, the call prrtc_robogpu.py:186-188 makes of the OptiX checker
+ * (check_edges_collision_free): waypoints [n_edges, granularity, dim]; out_free [n_edges] uint8 is
+ * 1 iff every waypoint of the edge is free of the world primitives and self pairs (the verdict of
+ * collision_mode, as prrtc_check_configs) and, when cloud is not NULL, of the cloud. */
+int prrtc_check_edges(prrtc_stream_t stream, const float* waypoints, int32_t n_edges, int32_t granularity, int32_t dim,
+                      const prrtc_robot* robot, const prrtc_world* world, int32_t collision_mode,
+                      float collision_margin, const prrtc_cloud* cloud, uint8_t* out_free, const prrtc_launch* launch,
+                      uint64_t* work_flops);
+
 /* Message of the last failing call on this thread. */
 const char* prrtc_last_error(void);
 

@@ -82,6 +82,8 @@ def bind(L: ctypes.CDLL) -> ctypes.CDLL:
     L.oc_in_collision_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)]
     L.oc_min_distance_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, _f32p]
     L.oc_sphere_positions.argtypes = [ctypes.POINTER(OcContext), _f32p, _f32p]
+    L.oc_cloud_hit_batch.argtypes = [ctypes.POINTER(OcContext), _f32p, ctypes.c_int, _f32p, ctypes.c_int,
+                                     ctypes.c_float, ctypes.POINTER(ctypes.c_uint8)]
     L.oc_halton.argtypes = [ctypes.c_int, ctypes.c_int]
     L.oc_halton.restype = ctypes.c_float
     L.oc_prime_order.argtypes = [ctypes.c_int, ctypes.c_uint, _i32p]
@@ -181,6 +183,16 @@ def in_collision(ctx: Context, qs, L=None) -> np.ndarray:
     qs = np.ascontiguousarray(np.asarray(qs, np.float32).reshape(-1, ctx.dim))
     out = np.zeros(qs.shape[0], np.uint8)
     (L or lib()).oc_in_collision_batch(ctypes.byref(ctx.c), _fp(qs), qs.shape[0], out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out.astype(bool)
+
+
+def cloud_hits(ctx: Context, qs, points, r_env: float) -> np.ndarray:
+    """Per configuration: some robot sphere within r + r_env of a cloud point (brute force)."""
+    qs = np.ascontiguousarray(np.asarray(qs, np.float32).reshape(-1, ctx.dim))
+    pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+    out = np.zeros(qs.shape[0], np.uint8)
+    lib().oc_cloud_hit_batch(ctypes.byref(ctx.c), _fp(qs), qs.shape[0], _fp(pts), pts.shape[0], float(r_env),
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
     return out.astype(bool)
 
 

@@ -457,6 +457,31 @@ void oc_min_distance_batch(const oc_context* c, const float* qs, int n, float* o
     for (int i = 0; i < n; ++i) out[i] = oc_min_distance(c, &qs[(size_t)i * c->dim]);
 }
 
+/* Point-cloud verdict of one configuration: some valid robot sphere s and cloud point p with
+ * |c_s - p| <= r_s + r_env, by hit_sphere with the point as a sphere of radius r_env (the
+ * definition the point-cloud edge checker uses, prrtc_robogpu.py:103-298).  Brute force over
+ * every point: no grid. */
+int oc_cloud_hit(const oc_context* c, const float* q, const float* points, int n_points, float r_env) {
+    oc_tf T[OC_MAX_JOINTS];
+    fk_sorted(c, q, T);
+    for (int s = 0; s < c->n_spheres; ++s) {
+        if (!link_ok(c, c->sphere_link_idx[s])) continue;
+        float p[3];
+        sphere_world(c, T, s, p);
+        const float r = c->sphere_radius[s];
+        for (int k = 0; k < n_points; ++k) {
+            const float o[4] = {points[3 * k], points[3 * k + 1], points[3 * k + 2], r_env};
+            if (hit_sphere(p[0], p[1], p[2], r, o)) return 1;
+        }
+    }
+    return 0;
+}
+
+void oc_cloud_hit_batch(const oc_context* c, const float* qs, int n, const float* points, int n_points, float r_env,
+                        uint8_t* out) {
+    for (int i = 0; i < n; ++i) out[i] = (uint8_t)oc_cloud_hit(c, &qs[(size_t)i * c->dim], points, n_points, r_env);
+}
+
 void oc_sphere_positions(const oc_context* c, const float* q, float* out /* [n_spheres,3] */) {
     oc_tf T[OC_MAX_JOINTS];
     fk_sorted(c, q, T);

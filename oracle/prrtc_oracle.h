@@ -76,6 +76,9 @@ void oc_in_collision_batch(const oc_context* c, const float* qs, int n, uint8_t*
 float oc_min_distance(const oc_context* c, const float* q);
 void oc_min_distance_batch(const oc_context* c, const float* qs, int n, float* out);
 void oc_sphere_positions(const oc_context* c, const float* q, float* out);
+int oc_cloud_hit(const oc_context* c, const float* q, const float* points, int n_points, float r_env);
+void oc_cloud_hit_batch(const oc_context* c, const float* qs, int n, const float* points, int n_points, float r_env,
+                        uint8_t* out);
 float oc_halton(int base, int index);
 void oc_prime_order(int explorer, unsigned seed, int* out16);
 void oc_nearest(const float* tree, int max_nodes, int size, int dim, const float* qs, int nq, int* idx, float* dsq);

@@ -7,6 +7,7 @@ kernels for gfx950 behind the C ABI in ``include/prrtc_amd.h``.
 
 from .prrtc import (PRRTCResult, CollisionChecker, forward_kinematics, prepare_batch, prrtc_extend,
                     prrtc_nearest_neighbor, prrtc_plan, prrtc_plan_batch)
+from .robogpu import PointCloud, PointCloudChecker, prrtc_plan_robogpu
 
 __version__ = "0.1.0"
 
@@ -19,4 +20,7 @@ __all__ = [
     "CollisionChecker",
     "forward_kinematics",
     "prepare_batch",
+    "prrtc_plan_robogpu",
+    "PointCloudChecker",
+    "PointCloud",
 ]

@@ -42,6 +42,9 @@ EXPORTED_SYMBOLS = (
     "prrtc_sphere_cost",
     "prrtc_refine_workspace_bytes",
     "prrtc_refine_paths",
+    "prrtc_cloud_bytes",
+    "prrtc_cloud_build",
+    "prrtc_check_edges",
     "prrtc_last_error",
     "prrtc_version",
 )
@@ -70,6 +73,14 @@ class World(ctypes.Structure):
         ("fine_link_offsets", _vp), ("coarse_self_pairs", _vp), ("self_pair_ranges", _vp),
         ("n_coarse_spheres", ctypes.c_int32), ("n_fine_link_offsets", ctypes.c_int32),
         ("n_coarse_self_pairs", ctypes.c_int32), ("n_self_pair_ranges", ctypes.c_int32),
+    ]
+
+
+class Cloud(ctypes.Structure):
+    _fields_ = [
+        ("points", _vp), ("cell_start", _vp), ("n_points", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("lo", ctypes.c_float * 3), ("cell", ctypes.c_float), ("r_env", ctypes.c_float),
     ]
 
 
@@ -147,6 +158,14 @@ def lib() -> ctypes.CDLL:
     L.prrtc_refine_paths.argtypes = [_vp, _vp, i32, i32, i32, _vp, _vp, P(Robot), P(World), i32, f32, f32, f32, f32,
                                      i32, _vp, _vp, sz, _vp]
     L.prrtc_refine_paths.restype = ctypes.c_int
+    f3 = ctypes.POINTER(ctypes.c_float)
+    L.prrtc_cloud_bytes.argtypes = [i32, f3, f3, f32]
+    L.prrtc_cloud_bytes.restype = sz
+    L.prrtc_cloud_build.argtypes = [_vp, _vp, i32, f3, f3, f32, f32, _vp, sz, P(Cloud)]
+    L.prrtc_cloud_build.restype = ctypes.c_int
+    L.prrtc_check_edges.argtypes = [_vp, _vp, i32, i32, i32, P(Robot), P(World), i32, f32, P(Cloud), _vp, P(Launch),
+                                    _vp]
+    L.prrtc_check_edges.restype = ctypes.c_int
     L.prrtc_last_error.argtypes = []
     L.prrtc_last_error.restype = ctypes.c_char_p
     return L

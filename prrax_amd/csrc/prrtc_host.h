@@ -26,5 +26,10 @@ RobotDev robot_dev(const prrtc_robot& r);
 WorldDev world_dev(const prrtc_world& w, bool per_query);
 int check_robot(const prrtc_robot* r);
 int check_world(const prrtc_world* w);
+// LDS layout of a check kernel's model with nt threads (shrinking nt until it fits), and the
+// offset of its scan scratch.
+int model_lds(const prrtc_robot& r, const prrtc_world& w, int& nt, int max_lds, ModelLayout& ml, size_t& lds,
+              bool per_lane);
+size_t scan_offset(const ModelLayout& ml);
 
 }  // namespace prrtc

@@ -58,8 +58,10 @@ def _resolve_collision_mode(collision_checker: str) -> int:
     except KeyError:
         if collision_checker == "robogpu":
             raise ValueError(
-                "collision_checker='robogpu' (OptiX point-cloud checking) is not available in the "
-                "in-kernel planner; use one of " + str(sorted(_COLLISION_MODES))
+                "collision_checker='robogpu' (point-cloud checking) is not available in the in-kernel "
+                "planner. Use the host-driven planner instead:\n"
+                "    from prrax_amd import prrtc_plan_robogpu, PointCloudChecker\n"
+                "which validates edges against the point cloud with prrtc_check_edges."
             ) from None
         raise ValueError(
             f"collision_checker must be one of {sorted(_COLLISION_MODES)}, got {collision_checker!r}"

@@ -356,3 +356,19 @@ def sample_free(rng, robot: Robot, n: int, is_free, batch: int = 256, max_tries:
 def query_seed(config: int, query: int) -> int:
     """Seed of start/goal sampling for query ``query`` of workload ``config``."""
     return 1000 * config + query
+
+
+def scene_cloud(n_points: int = 100_000, robot_seed: int = 0, world_seed: int = 1, r_env: float = 0.02) -> Scene:
+    """Point-cloud scene: the C1 arm, the C1 world's 40 spheres sampled as ``n_points`` surface
+    points (uniform over the total area; a dense capture of the same objects), and the floor
+    halfspace as the analytic world.  ``extra`` holds ``points`` [n, 3] and ``r_env``."""
+    sc = scene_c1(robot_seed, world_seed)
+    rng = np.random.default_rng(world_seed + 7000)
+    ws = sc.world["world_spheres"].astype(np.float64)
+    area = ws[:, 3] ** 2
+    k = rng.choice(ws.shape[0], size=n_points, p=area / area.sum())
+    d = rng.normal(size=(n_points, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pts = ws[k, :3] + d * ws[k, 3:4]
+    world = {"world_halfspaces": sc.world["world_halfspaces"]}
+    return Scene(sc.robot, world, extra={"points": pts.astype(np.float32), "r_env": float(r_env)})

@@ -229,3 +229,56 @@ def test_simt_harness_catches_planted_mistakes(tmp_path, mode, message):
         assert r.returncode == 0, r.stderr
     else:
         assert r.returncode != 0 and message in r.stderr, r.stderr[-2000:]
+
+
+@pytest.fixture(scope="module")
+def simt_cloud_exe(tmp_path_factory) -> Path:
+    exe = tmp_path_factory.mktemp("simtc") / "simt_cloud"
+    cmd = ["g++", "-std=c++20", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined", "-ffp-contract=off", "-Wall", "-Wno-attributes", "-Wno-unknown-pragmas",
+           "-I", str(SIMT), "-I", str(ROOT / "prrax_amd" / "csrc"), "-I", str(ROOT / "include"),
+           str(SIMT / "simt_cloud.cpp"), "-o", str(exe), "-lpthread"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def run_simt_cloud(exe, tmp, cc, dim, q, points, r_env, lo, cell, shape, mode=0, margin=0.0):
+    A = check_context(cc, dim)
+    q = np.ascontiguousarray(q, np.float32)
+    n = q.shape[0]
+    inp, out = tmp / "cin.bin", tmp / "cout.bin"
+    _write(inp, [A["fk_twists"], A["fk_parent_tf"], A["fk_parent_idx"], A["fk_act_idx"], A["fk_mimic_mul"],
+                 A["fk_mimic_off"], A["fk_mimic_act_idx"], A["fk_topo_inv"], A["sphere_link_idx"], A["sphere_local"],
+                 A["sphere_radius"], A["world_spheres"], A["world_capsules"], A["world_boxes"],
+                 A["world_halfspaces"], A["self_pairs"], np.array([dim, mode, n, *shape], np.int32),
+                 np.array([margin, *lo, cell, r_env], np.float32), np.asarray(points, np.float32), q])
+    r = subprocess.run([str(exe), str(inp), str(out)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, f"simt_cloud failed ({r.returncode}):\n{r.stderr[-4000:]}"
+    raw = out.read_bytes()
+    return np.frombuffer(raw, np.uint8, n, 0).astype(bool), np.frombuffer(raw, np.uint8, n, n).astype(bool)
+
+
+@pytest.mark.parametrize("grid", ["fit", "fine", "one_cell", "clipped"])
+def test_simt_cloud_grid_and_verdicts(simt_cloud_exe, tmp_path, grid):
+    """wave_cloud_hit against a host-built grid: the same verdict as the oracle's brute force over
+    every point, for the default cell (twice the largest reach), cells much smaller than a reach
+    (many rows per query), a single cell, and bounds smaller than the cloud (points clamped into
+    border cells)."""
+    sc = scenes.scene_cloud(6000)
+    pts, r_env = sc.extra["points"], sc.extra["r_env"]
+    cc = sc.context()
+    rng = np.random.default_rng(12)
+    q = np.concatenate([edge_configs(rng, sc.robot.lower, sc.robot.upper, 10, 0.05),
+                        rng.uniform(sc.robot.lower, sc.robot.upper, size=(64, 7)).astype(np.float32)])
+    lo, hi = pts.min(0), pts.max(0)
+    reach = float(np.max(cc["sphere_radius"])) + r_env
+    cell = {"fit": 2 * reach, "fine": 0.3 * reach, "one_cell": float((hi - lo).max()) + 1.0,
+            "clipped": 2 * reach}[grid]
+    if grid == "clipped":
+        lo, hi = lo + 0.3 * (hi - lo), hi - 0.3 * (hi - lo)
+    shape = [max(1, int(np.ceil((h - l) / cell))) for l, h in zip(lo, hi)]
+    hw, hc = run_simt_cloud(simt_cloud_exe, tmp_path, cc, 7, q, pts, r_env, lo, cell, shape)
+    np.testing.assert_array_equal(hw, O.in_collision(O.Context(cc, 7), q))
+    want = O.cloud_hits(O.Context(cc, 7), q, pts, r_env)
+    np.testing.assert_array_equal(hc, want)
+    assert 0.05 < want.mean() < 0.95
