@@ -342,6 +342,110 @@ def fkcc_leg(n_cfg: int) -> dict:
                         "16 boxes, 8 capsules, floor)", **fkcc}
 
 
+def cloud_edges(sc, n_edges: int, rng) -> np.ndarray:
+    """Edges [n_edges, 16, 7]: random starts in the limits, steps of about 0.3 rad."""
+    a = rng.uniform(sc.robot.lower, sc.robot.upper, size=(n_edges, 1, 7))
+    b = a + rng.normal(scale=0.3 / np.sqrt(7.0), size=a.shape)
+    t = np.linspace(0.0, 1.0, 16)[None, :, None]
+    return (a * (1.0 - t) + b * t).astype(np.float32)
+
+
+def cloud_queries(n_queries: int, chk, sc, pcl):
+    from prrax_amd import scenes
+
+    def free(q):
+        return chk.edges_free(np.asarray(q, np.float32)[:, None, :], sc.world, pcl, sc.extra["r_env"]).cpu().numpy() == 1
+
+    return [scenes.sample_free(np.random.default_rng(scenes.query_seed(7, qi)), sc.robot, 2, free)
+            for qi in range(n_queries)]
+
+
+def cloud_leg(n_points: int, n_queries: int, n_edges: int = 65536) -> dict:
+    """Point-cloud path: the grid build, edge validation against an n_points cloud (the points of
+    the C1 world's spheres' surfaces) plus the floor, and the host-driven planner."""
+    import torch
+
+    from prrax_amd import PointCloud, PointCloudChecker, prrtc_plan_robogpu, scenes
+
+    sc = scenes.scene_cloud(n_points)
+    pts, r_env = sc.extra["points"], sc.extra["r_env"]
+    chk = PointCloudChecker(sc.context(), 7)
+    pcl = torch.from_numpy(pts).cuda()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    PointCloud(pcl, r_env, chk.max_radius)
+    torch.cuda.synchronize()
+    e0.record()
+    grid = PointCloud(pcl, r_env, chk.max_radius)
+    e1.record()
+    torch.cuda.synchronize()
+    build_ms = e0.elapsed_time(e1)
+    rng = np.random.default_rng(13)
+    wp = torch.from_numpy(cloud_edges(sc, n_edges, rng)).cuda()
+    flops = torch.zeros(1, dtype=torch.int64, device="cuda")
+    chk.edges_free(wp, sc.world, pcl, r_env)
+    torch.cuda.synchronize()
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        out = chk.edges_free(wp, sc.world, pcl, r_env, flops)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    free_frac = float(out.float().mean().item())
+    tf = float(flops.item()) / reps / (ms / 1000.0) / 1e12
+    queries = cloud_queries(n_queries, chk, sc, pcl)
+    kw = dict(max_iterations=2000, step_size=0.5, num_new_samples=128, granularity=16, max_nodes=100_000,
+              connect_max_steps=32)
+    times, solved, kernel_ms = [], [], []
+    for qi, sg in enumerate(queries):
+        t0 = time.perf_counter()
+        r = prrtc_plan_robogpu(sg[0], sg[1:], sc.robot, chk, world_geom=sc.world, point_cloud=pcl, r_env=r_env,
+                               seed=qi, **kw)
+        times.append(time.perf_counter() - t0)
+        solved.append(r.solved)
+        kernel_ms.append(r.kernel_time_ms)
+    out = {"workload": f"C1 arm (48 spheres) vs a {n_points}-point cloud (surfaces of 40 spheres, r_env {r_env}) "
+                       f"+ floor; {n_edges} edges x 16 waypoints; {n_queries} host-driven plans "
+                       f"(S {kw['num_new_samples']}, G 16, max_iterations {kw['max_iterations']})",
+           "grid": {"cells": int(grid.struct.nx * grid.struct.ny * grid.struct.nz), "cell_m": round(grid.cell, 4),
+                    "build_ms": round(build_ms, 3)},
+           "check_edges": {"ms": round(ms, 3), "edges_per_s": round(n_edges / (ms / 1000.0), 1),
+                           "cfg_per_s": round(16 * n_edges / (ms / 1000.0), 1), "tflops": round(tf, 3),
+                           "free_fraction": round(free_frac, 4)},
+           "plan": {"plans_per_s": round(len(times) / sum(times), 3), "p50_ms_per_plan": round(1000 * float(np.median(times)), 1),
+                    "solve_rate": round(float(np.mean(solved)), 4),
+                    "checker_share": round(float(sum(kernel_ms)) / (1000.0 * sum(times)), 4)}}
+    log(f"cloud: {out}")
+    return out, (sc, queries, kw)
+
+
+def _cloud_cpu_job(job):
+    from oracle import host_rrtc
+
+    cc, pts, r_env, start, goals, kw, limit = job
+    t = time.perf_counter()
+    r = host_rrtc.plan(host_rrtc.CloudChecker(cc, len(start), pts, r_env), start, goals, time_limit=limit, **kw)
+    return {"solved": bool(r["solved"]), "timed_out": bool(r["timed_out"]), "s": time.perf_counter() - t}
+
+
+def cpu_cloud_leg(pool, workers: int, setup, n: int, limit: float) -> dict:
+    """The same host loop with a CPU checker (float32 NumPy FK + a scipy k-d tree over the
+    points), on the first n of the GPU leg's queries, one per worker process."""
+    sc, queries, kw = setup
+    cc = sc.context()
+    jobs = [(cc, sc.extra["points"], sc.extra["r_env"], sg[0], sg[1:],
+             dict(kw, min_vals=sc.robot.lower, max_vals=sc.robot.upper, seed=qi), limit)
+            for qi, sg in enumerate(queries[:n])]
+    t0 = time.perf_counter()
+    res = pool.map(_cloud_cpu_job, jobs, chunksize=1)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(jobs) / dt, 3), "unit": "plans/s", "cores": workers, "queries": len(jobs),
+            "p50_s_per_plan": round(float(np.median([r["s"] for r in res])), 3),
+            "solve_rate": round(float(np.mean([r["solved"] for r in res])), 4),
+            "time_limited": int(sum(r["timed_out"] for r in res)),
+            "checker": "float32 NumPy FK + scipy cKDTree nearest point per robot sphere", "wall_s": round(dt, 2)}
+
+
 # ------------------------------------------------------------------------------ CPU baselines
 def _one_thread():
     from threadpoolctl import threadpool_limits
@@ -458,6 +562,8 @@ def main():
     ap.add_argument("--dual-arm-queries", type=int, default=256, help="queries of the 14-DOF leg (0 = skip)")
     ap.add_argument("--scale-queries", type=int, default=4096,
                     help="queries of the batch-size scaling leg (same generator, one launch; 0 = skip)")
+    ap.add_argument("--cloud-points", type=int, default=100_000, help="points of the point-cloud leg (0 = skip)")
+    ap.add_argument("--cloud-queries", type=int, default=16, help="host-driven plans of the point-cloud leg")
     ap.add_argument("--no-validate", action="store_true", help="skip the FP64 re-validation of the paths")
     ap.add_argument("--no-autotune", action="store_true", help="use S = --samples and the library's launch defaults")
     ap.add_argument("--no-cpu", action="store_true")
@@ -552,6 +658,7 @@ def main():
     scale = scale_leg(args.scale_queries, kw, launch) if args.scale_queries > 0 else None
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
+    cloud, cloud_setup = cloud_leg(args.cloud_points, args.cloud_queries) if args.cloud_points > 0 else (None, None)
     achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
     shape_now = dict(stats.get("launch", {}), num_new_samples=kw["num_new_samples"])
     same_shape = args.queries == 1024 and all(shape_now.get(k) == v for k, v in TRAFFIC["shape"].items())
@@ -592,6 +699,7 @@ def main():
         "batch_scaling": scale,
         "refine": refine,
         "dual_arm": dual,
+        "point_cloud": cloud,
         "roofline": {"kernel": "plan_pass1+plan_pass2", "bound": "valu", "achieved": round(achieved, 4),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                      "traffic": TRAFFIC["bytes_per_step"] if same_shape else None,
@@ -618,6 +726,10 @@ def main():
         even = [i for i in sample if i not in set(order)]
         out["cpu_lockstep_oracle"] = cpu_lockstep_leg(worlds, starts, goals, nodes, kw, even, workers)
         log(f"cpu lockstep oracle: {out['cpu_lockstep_oracle']}")
+        if cloud_setup is not None:
+            out["cpu_point_cloud"] = cpu_cloud_leg(pool, workers, cloud_setup, min(workers, args.cloud_queries),
+                                                   args.cpu_limit)
+            log(f"cpu point cloud: {out['cpu_point_cloud']}")
         pool.close()
         pool.join()
     print(json.dumps(out), flush=True)

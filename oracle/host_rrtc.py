@@ -104,10 +104,14 @@ class BatchChecker:
         """True for each configuration with no pair distance <= 0."""
         qs = np.asarray(qs, F32).reshape(-1, self.dim)
         n = qs.shape[0]
-        hit = np.zeros(n, bool)
         if not self.enabled or n == 0:
-            return ~hit
-        P = self._centres(qs)
+            return np.ones(n, bool)
+        return ~self._hits(self._centres(qs))
+
+    def _hits(self, P: np.ndarray) -> np.ndarray:
+        """Contact of the sphere centres P [n, spheres, 3] with the world or a self pair."""
+        n = P.shape[0]
+        hit = np.zeros(n, bool)
         r = self.rad
         if self.ws.shape[0]:
             # |p - o|^2 = |p|^2 + |o|^2 - 2 p.o: one matrix product, evaluated in float64 so that
@@ -134,6 +138,35 @@ class BatchChecker:
             d2 = ((P[:, self.pi] - P[:, self.pj]) ** 2).sum(-1)
             rr = r[self.pi] + r[self.pj]
             hit |= ((rr >= 0)[None] & (d2 <= (rr * rr)[None])).any(axis=1)
+        return hit
+
+
+class CloudChecker(BatchChecker):
+    """BatchChecker plus an environment point cloud, each point a sphere of radius ``r_env``
+    (the point-cloud planner's checker, prrtc_robogpu.py:103-298): a k-d tree over the points
+    (scipy) gives every robot sphere's nearest point in float64, and a sphere within
+    r + r_env of it is in contact."""
+
+    def __init__(self, cc: dict, dim: int, points, r_env: float):
+        super().__init__(cc, dim)
+        from scipy.spatial import cKDTree
+
+        pts = np.asarray(points, np.float64).reshape(-1, 3)
+        self.tree = cKDTree(pts) if pts.shape[0] else None
+        self.r_env = float(r_env)
+
+    def free(self, qs: np.ndarray) -> np.ndarray:
+        qs = np.asarray(qs, F32).reshape(-1, self.dim)
+        n = qs.shape[0]
+        if n == 0 or self.par.shape[0] == 0 or self.rad.shape[0] == 0:
+            return np.ones(n, bool)
+        P = self._centres(qs)
+        hit = self._hits(P) if self.enabled else np.zeros(n, bool)
+        if self.tree is not None:
+            reach = self.rad.astype(np.float64) + self.r_env
+            d, _ = self.tree.query(P.reshape(-1, 3).astype(np.float64), k=1,
+                                   distance_upper_bound=float(reach.max()) * (1 + 1e-9) + 1e-12)
+            hit |= (d.reshape(n, -1) <= reach[None]).any(axis=1)
         return ~hit
 
 

@@ -113,6 +113,7 @@ struct EdgeArgs {
     float margin;
     CloudDev cloud;
     int has_cloud;
+    int tile;                // waypoints per wave batch (FK one per lane), 1..kFB
     uint8_t* out_free;       // [n_edges], 1 on entry
     unsigned long long* flops;
 };
@@ -121,9 +122,10 @@ __global__ void edge_init_kernel(uint8_t* out_free, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out_free[i] = 1;
 }
 
-// Waves take batches of kFB waypoints (FK one per lane), then check them one by one with the
+// Waves take batches of `tile` waypoints (FK one per lane), then check them one by one with the
 // whole wave: the world and self pairs (the planner's culled verdict), then the cloud.  An edge
-// already found colliding skips its remaining waypoints.
+// already found colliding skips its remaining waypoints.  Small calls (a planner round's edges)
+// take tiles of fewer waypoints, so that they still spread over the GPU's waves.
 __global__ __launch_bounds__(1024) void edge_check_kernel(EdgeArgs a) {
     extern __shared__ __align__(16) char lds[];
     int* scan = reinterpret_cast<int*>(lds + a.off_scan);
@@ -138,18 +140,19 @@ __global__ __launch_bounds__(1024) void edge_check_kernel(EdgeArgs a) {
     const int wpb = blockDim.x / kWave;
     const int total_waves = gridDim.x * wpb;
     const long long n = static_cast<long long>(a.n_edges) * a.G;
-    const long long nb = (n + kFB - 1) / kFB;
+    const int tile = a.tile;
+    const long long nb = (n + tile - 1) / tile;
     unsigned long long flops = 0;
     for (long long b = static_cast<long long>(blockIdx.x) * wpb + threadIdx.x / kWave; b < nb; b += total_waves) {
-        const long long i = b * kFB + lane;
-        if (lane < kFB && i < n) {
+        const long long i = b * tile + lane;
+        if (lane < tile && i < n) {
             for (int d = 0; d < a.dim; ++d) L.q[d * kFBP] = a.waypoints[i * a.dim + d];
             if (M.n_joints > 0) fk_lane(M, L);
         }
         wave_sync();
-        const int nc = static_cast<int>(min(static_cast<long long>(kFB), n - b * kFB));
+        const int nc = static_cast<int>(min(static_cast<long long>(tile), n - b * tile));
         for (int c = 0; c < nc; ++c) {
-            const int e = static_cast<int>((b * kFB + c) / a.G);
+            const int e = static_cast<int>((b * tile + c) / a.G);
             // lane 0's reading of the edge's flag decides for the wave
             const int open = __builtin_amdgcn_readfirstlane(
                 static_cast<int>(*reinterpret_cast<volatile const uint8_t*>(a.out_free + e)));
@@ -316,7 +319,12 @@ int prrtc_check_edges(prrtc_stream_t stream, const float* waypoints, int32_t n_e
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(edge_init_kernel, dim3(std::max(1, std::min((n_edges + 255) / 256, 4096))), dim3(256), 0, st,
                        out_free, n_edges);
-    const long long batches = (static_cast<long long>(n_edges) * granularity + kFB - 1) / kFB;
+    // waypoints per wave batch: the largest power of two <= kFB that gives every CU 16 batches
+    const long long nwp = static_cast<long long>(n_edges) * granularity;
+    int tile = kFB;
+    while (tile > 1 && (nwp + tile - 1) / tile < 16ll * di.cus) tile /= 2;
+    a.tile = tile;
+    const long long batches = (nwp + tile - 1) / tile;
     long long grid = (launch && launch->slots > 0) ? launch->slots : (batches + nt / kWave - 1) / (nt / kWave);
     grid = std::max(1ll, std::min(grid, static_cast<long long>(8 * di.cus)));
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_check_kernel),

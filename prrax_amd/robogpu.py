@@ -34,9 +34,9 @@ MAX_CELLS = 1 << 22
 class PointCloud:
     """An environment cloud ``[n, 3]`` in the library's device grid.
 
-    The cell edge defaults to twice the largest robot-sphere reach (radius + ``r_env``), enlarged
-    until the grid has at most 2^22 cells, so that a sphere's query spans at most two cells per
-    axis."""
+    The cell edge defaults to half the largest robot-sphere reach (radius + ``r_env``), enlarged
+    until the grid has at most 2^22 cells: a sphere's query then visits about five rows of cells
+    per axis, each clipped to the sphere's chord, so the points it tests lie close to its reach."""
 
     def __init__(self, points, r_env: float, max_robot_radius: float, lo=None, hi=None, cell: Optional[float] = None):
         dev = _device()
@@ -53,7 +53,7 @@ class PointCloud:
         self.lo = np.ascontiguousarray(lo, np.float32)
         self.hi = np.ascontiguousarray(np.maximum(hi, lo), np.float32)
         if cell is None:
-            cell = max(2.0 * (float(max_robot_radius) + float(r_env)), 1e-3)
+            cell = max(0.5 * (float(max_robot_radius) + float(r_env)), 1e-3)
             ext = np.maximum(self.hi - self.lo, 0.0).astype(np.float64)
             while np.prod(np.maximum(np.ceil(ext / cell), 1.0)) > MAX_CELLS:
                 cell *= 1.25

@@ -15,10 +15,10 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
-#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <thread>
 
 #define __device__
 #define __host__
@@ -60,11 +60,11 @@ namespace simt {
 constexpr int kLanes = 64;
 
 // Rendezvous of the 64 lanes: each lane deposits a 64-bit word, and all lanes see all words.
+// Arrivals count on an atomic; the last one checks the lines and opens the next generation, the
+// others spin (yielding the CPU) until it does, or abort after timeout_s seconds.
 struct Wave {
-    std::mutex mu;
-    std::condition_variable cv;
-    int arrived = 0;
-    unsigned long long generation = 0;
+    std::atomic<int> arrived{0};
+    std::atomic<unsigned long long> generation{0};
     unsigned long long words[2][kLanes] = {};
     int lines[2][kLanes] = {};
     int timeout_s = 20;
@@ -75,24 +75,30 @@ int lane();
 // Deposit `word` (tag: the source line), wait for every lane, return the buffer of all words.
 inline const unsigned long long* exchange(unsigned long long word, int line) {
     Wave& W = wave();
-    std::unique_lock<std::mutex> lk(W.mu);
-    const int buf = static_cast<int>(W.generation & 1ull);
+    const unsigned long long gen = W.generation.load(std::memory_order_acquire);
+    const int buf = static_cast<int>(gen & 1ull);
     W.words[buf][lane()] = word;
     W.lines[buf][lane()] = line;
-    const unsigned long long gen = W.generation;
-    if (++W.arrived == kLanes) {
+    if (W.arrived.fetch_add(1, std::memory_order_acq_rel) + 1 == kLanes) {
         for (int l = 1; l < kLanes; ++l)
             if (W.lines[buf][l] != W.lines[buf][0]) {
                 std::fprintf(stderr, "simt: divergent collective: lane 0 at line %d, lane %d at line %d\n",
                              W.lines[buf][0], l, W.lines[buf][l]);
                 std::abort();
             }
-        W.arrived = 0;
-        ++W.generation;
-        W.cv.notify_all();
-    } else if (!W.cv.wait_for(lk, std::chrono::seconds(W.timeout_s), [&] { return W.generation != gen; })) {
-        std::fprintf(stderr, "simt: collective at line %d: only %d of %d lanes arrived\n", line, W.arrived, kLanes);
-        std::abort();
+        W.arrived.store(0, std::memory_order_relaxed);
+        W.generation.store(gen + 1, std::memory_order_release);
+    } else {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned spins = 0; W.generation.load(std::memory_order_acquire) == gen; ++spins) {
+            std::this_thread::yield();
+            if ((spins & 4095u) == 4095u &&
+                std::chrono::steady_clock::now() - t0 > std::chrono::seconds(W.timeout_s)) {
+                std::fprintf(stderr, "simt: collective at line %d: only %d of %d lanes arrived\n", line,
+                             W.arrived.load(), kLanes);
+                std::abort();
+            }
+        }
     }
     // Buffer `buf` is rewritten only by the collective after next, which no lane can enter
     // before every lane has left this one.

@@ -53,3 +53,23 @@ def test_cloud_matches_fp64_away_from_contact():
     far = np.abs(margin) > 1e-5
     np.testing.assert_array_equal(got[far], margin[far] <= 0)
     assert far.mean() > 0.95 and 0.05 < got.mean() < 0.95
+
+
+def test_host_cloud_checker_matches_oracle_away_from_contact():
+    """The CPU baseline's k-d tree checker (oracle.host_rrtc.CloudChecker) agrees with the
+    oracle's brute force wherever both the world and the cloud are decidable in float32."""
+    from oracle import host_rrtc
+
+    sc = scenes.scene_cloud(4000)
+    pts, r_env = sc.extra["points"], sc.extra["r_env"]
+    cc = sc.context()
+    ctx = O.Context(cc, 7)
+    rng = np.random.default_rng(8)
+    q = rng.uniform(sc.robot.lower, sc.robot.upper, size=(400, 7)).astype(np.float32)
+    want = ~(O.in_collision(ctx, q) | O.cloud_hits(ctx, q, pts, r_env))
+    got = host_rrtc.CloudChecker(cc, 7, pts, r_env).free(q)
+    world_md = np.abs(O.min_distance(O.Context(cc, 7, collision_mode=1, collision_margin=0.0), q))
+    shrunk = O.cloud_hits(ctx, q, pts, r_env - 1e-5) == O.cloud_hits(ctx, q, pts, r_env + 1e-5)
+    ok = (world_md > 1e-5) & shrunk
+    assert ok.mean() > 0.9
+    np.testing.assert_array_equal(got[ok], want[ok])

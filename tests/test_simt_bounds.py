@@ -258,12 +258,12 @@ def run_simt_cloud(exe, tmp, cc, dim, q, points, r_env, lo, cell, shape, mode=0,
     return np.frombuffer(raw, np.uint8, n, 0).astype(bool), np.frombuffer(raw, np.uint8, n, n).astype(bool)
 
 
-@pytest.mark.parametrize("grid", ["fit", "fine", "one_cell", "clipped"])
+@pytest.mark.parametrize("grid", ["fit", "half", "fine", "one_cell", "clipped"])
 def test_simt_cloud_grid_and_verdicts(simt_cloud_exe, tmp_path, grid):
     """wave_cloud_hit against a host-built grid: the same verdict as the oracle's brute force over
-    every point, for the default cell (twice the largest reach), cells much smaller than a reach
-    (many rows per query), a single cell, and bounds smaller than the cloud (points clamped into
-    border cells)."""
+    every point, for cells of twice and half the largest reach (the default), cells much smaller
+    than a reach (many rows per query), a single cell, and bounds smaller than the cloud (points
+    clamped into border cells)."""
     sc = scenes.scene_cloud(6000)
     pts, r_env = sc.extra["points"], sc.extra["r_env"]
     cc = sc.context()
@@ -272,7 +272,7 @@ def test_simt_cloud_grid_and_verdicts(simt_cloud_exe, tmp_path, grid):
                         rng.uniform(sc.robot.lower, sc.robot.upper, size=(64, 7)).astype(np.float32)])
     lo, hi = pts.min(0), pts.max(0)
     reach = float(np.max(cc["sphere_radius"])) + r_env
-    cell = {"fit": 2 * reach, "fine": 0.3 * reach, "one_cell": float((hi - lo).max()) + 1.0,
+    cell = {"fit": 2 * reach, "half": 0.5 * reach, "fine": 0.3 * reach, "one_cell": float((hi - lo).max()) + 1.0,
             "clipped": 2 * reach}[grid]
     if grid == "clipped":
         lo, hi = lo + 0.3 * (hi - lo), hi - 0.3 * (hi - lo)
