@@ -604,6 +604,14 @@ __device__ __forceinline__ bool self_pairs_mode(const Model& M, bool mg, const f
     return mg ? self_pairs_2<true>(M, FP, k0, lane) : self_pairs_2<false>(M, FP, k0, lane);
 }
 
+// Lanes per queued entry of a refinement of qn entries (wave-uniform): the largest power of two
+// lpe <= 64 with qn * lpe <= 64, 1 when qn > 32.
+__device__ __forceinline__ int lanes_per_entry(int qn) {
+    int lpe = 1;
+    while (lpe < kWave && qn * lpe * 2 <= kWave) lpe *= 2;
+    return lpe;
+}
+
 __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned long long& flops) {
     const int lane = threadIdx.x & (kWave - 1);
     char* area = M.cull_base + static_cast<size_t>(threadIdx.x / kWave) * M.cull_stride;
@@ -658,13 +666,17 @@ __device__ bool wave_culled(const Model& M, const float* T, int ts, unsigned lon
     // Refine as soon as one pair is queued, then wait for twice as many after each clear
     // refinement: a colliding configuration exits early, a free one refines in full batches.
     int flush_at = 4;
+    // A refinement gives each queued pair lpe lanes (the largest power of two with qn * lpe <= 64),
+    // which split the group's members: small early batches keep the wave busy too.
     auto refine = [&]() -> bool {
         wave_sync();
         bool h = false;
-        for (int e = lane; e < qn && !h; e += kWave) {
+        const int lpe = lanes_per_entry(qn);
+        const int sub = lane & (lpe - 1);
+        for (int e = lane / lpe; e < qn && !h; e += kWave / lpe) {
             const int o = Q[e] >> 6, g = Q[e] & 63;
             PRRTC_SIMT_ASSERT(e < kCullQ && o < M.n_wh + M.n_wb + M.n_wc + M.n_ws && g < ng);
-            for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
+            for (int m = M.goff[g] + sub; m < M.goff[g + 1]; m += lpe) {
                 if (fine_test_mode(M, mg, o, FP[M.gmem[m]], lf)) {
                     h = true;
                     break;
@@ -903,13 +915,15 @@ __device__ int wave_unit_check(const Model& M, const float* T, int ts, int nu, u
         const int qn = static_cast<int>(__popcll(mask));  // <= kWave <= kCullQ
         PRRTC_SIMT_OBSERVE(2, qn);
         wave_sync();
-        for (int e = lane; e < qn; e += kWave) {
+        const int lpe = lanes_per_entry(qn);
+        const int sub = lane & (lpe - 1);
+        for (int e = lane / lpe; e < qn; e += kWave / lpe) {
             const int o = Q[e] >> 10, u = (Q[e] >> 6) & 15, g = Q[e] & 63;
             PRRTC_SIMT_ASSERT(e < kCullQ && u < nu && g < ng && o < M.n_wh + M.n_wb + M.n_wc + M.n_ws);
             Mat3 R;
             float tx, ty, tz;
             frame_from(T + u, ts, M.gframe[g], R, tx, ty, tz);
-            for (int m = M.goff[g]; m < M.goff[g + 1]; ++m) {
+            for (int m = M.goff[g] + sub; m < M.goff[g + 1]; m += lpe) {
                 const float4 sp = M.spheres[M.gmem[m]];
                 const float3 p = mat_apply(R, tx, ty, tz, sp.x, sp.y, sp.z);
                 lf += kFlopsXf;
