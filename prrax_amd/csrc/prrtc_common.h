@@ -40,7 +40,7 @@ struct WorldDev {
 // wq + w * kMaxDim * fbp floats (configurations, element c of dim d at [d * fbp + c]) and
 // wT + w * nj * 7 * fbp floats (joint poses, element c of slot k at [k * fbp + c]).
 struct ModelLayout {
-    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, gc, gf, go, gm, wq, wT, wcull, end;
+    size_t joints, spheres, sphere_js, pairs, wsph, wcap, wbox, whs, gc, gf, go, gm, fg, wq, wT, wcull, end;
     int fbp, cull_stride;
 };
 
@@ -60,6 +60,7 @@ __host__ __device__ inline ModelLayout model_layout(int nj, int nrs, int nsp, in
     L.gf = o; o = align16(o + sizeof(int) * nj);
     L.go = o; o = align16(o + sizeof(int) * (nj + 1));
     L.gm = o; o = align16(o + sizeof(int) * nrs);
+    L.fg = o; o = align16(o + sizeof(int) * nj);
     L.wq = o; o = align16(o + sizeof(float) * kMaxDim * fbp * nwaves);
     L.wT = o; o = align16(o + sizeof(float) * 7 * nj * fbp * nwaves);
     L.cull_stride = static_cast<int>(align16(static_cast<size_t>(cull_stride_bytes(nrs, nj, ucap))));
@@ -275,6 +276,7 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
     int* GF = reinterpret_cast<int*>(lds + Lo.gf);
     int* GO = reinterpret_cast<int*>(lds + Lo.go);
     int* GM = reinterpret_cast<int*>(lds + Lo.gm);
+    int* FG = reinterpret_cast<int*>(lds + Lo.fg);
     int cnt = 0;
     if (tid < nj)
         for (int s2 = 0; s2 < nrs; ++s2) cnt += (SJ[s2] == tid) ? 1 : 0;
@@ -303,7 +305,8 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
         GO[gid] = goff;
     }
     if (tid == 0) GO[n_groups] = n_members;
-    M.gcoarse = GC; M.gframe = GF; M.goff = GO; M.gmem = GM; M.n_groups = n_groups;
+    if (tid < nj) FG[tid] = (cnt > 0) ? gid : -1;
+    M.gcoarse = GC; M.gframe = GF; M.goff = GO; M.gmem = GM; M.fgroup = FG; M.n_groups = n_groups;
     M.cull_base = lds + Lo.wcull;
     M.cull_stride = Lo.cull_stride;
     M.joints = J; M.spheres = SP; M.sphere_js = SJ; M.pairs = PR;

@@ -282,3 +282,18 @@ def test_simt_cloud_grid_and_verdicts(simt_cloud_exe, tmp_path, grid):
     want = O.cloud_hits(O.Context(cc, 7), q, pts, r_env)
     np.testing.assert_array_equal(hc, want)
     assert 0.05 < want.mean() < 0.95
+
+
+@pytest.mark.parametrize("robot", ["arm", "dual"])
+def test_simt_self_pairs_only(simt_exe, tmp_path, robot):
+    """No world: every unit verdict comes from the self-pair stage (group-pair culling, the
+    survivor list, and for the dual arm's 2000 pairs the fall-back over all pairs)."""
+    sc = scenes.scene_c2() if robot == "arm" else scenes.scene_c4()
+    cc = dict(sc.robot.ctx)
+    dim = sc.robot.dim
+    rng = np.random.default_rng(31)
+    q = np.concatenate([edge_configs(rng, sc.robot.lower, sc.robot.upper, 12, 0.1),
+                        rng.uniform(sc.robot.lower, sc.robot.upper, size=(64, dim)).astype(np.float32)])
+    hits = (0.02, 0.98) if robot == "arm" else (0.0, 1.0)  # the two arms overlap in most configurations
+    check_case(simt_exe, tmp_path, cc, dim, q, 0, 0.0, expect_hits=hits)
+    check_case(simt_exe, tmp_path, cc, dim, q, 1, 0.01, expect_hits=hits)
