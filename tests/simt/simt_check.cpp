@@ -14,8 +14,8 @@
 // [n_configs * dim], units i32 [n_units * 2] (first configuration, count; a unit lies inside one
 // batch of 16 consecutive configurations).
 // Output: hit_culled u8[n], min_distance f32[n], unit verdict i32[n_units],
-// observed maxima i64[4] (culled queue depth, unit candidate list length, unit refinement queue
-// depth, groups).
+// observed maxima i64[6] (culled queue depth, unit candidate list length, unit refinement queue
+// depth, groups, surviving self pairs of a unit, units whose survivors overflowed the list).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -34,10 +34,14 @@ simt_dim3 gridDim;
 namespace simt {
 static Wave g_wave;
 static thread_local int g_lane = 0;
-static std::atomic<long long> g_observed[4];
+static std::atomic<long long> g_observed[6];
 Wave& wave() { return g_wave; }
 int lane() { return g_lane; }
 void observe(int slot, long long value) {
+    if (slot == 5) {  // a count, not a maximum
+        g_observed[5] += value;
+        return;
+    }
     long long cur = g_observed[slot].load();
     while (value > cur && !g_observed[slot].compare_exchange_weak(cur, value)) {
     }
@@ -176,7 +180,7 @@ int main(int argc, char** argv) {
     out.write(reinterpret_cast<const char*>(hit_culled.data()), n);
     out.write(reinterpret_cast<const char*>(min_d.data()), static_cast<std::streamsize>(4 * n));
     out.write(reinterpret_cast<const char*>(unit_v.data()), static_cast<std::streamsize>(4 * nun));
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < 6; ++s) {
         const long long v = simt::g_observed[s].load();
         out.write(reinterpret_cast<const char*>(&v), 8);
     }

@@ -71,10 +71,10 @@ def run_simt(exe: Path, tmp: Path, cc: dict, dim: int, q: np.ndarray, units: np.
     hit_culled = np.frombuffer(raw, np.uint8, n, o); o += n
     md = np.frombuffer(raw, np.float32, n, o); o += 4 * n
     uv = np.frombuffer(raw, np.int32, nu, o); o += 4 * nu
-    obs = np.frombuffer(raw, np.int64, 4, o)
+    obs = np.frombuffer(raw, np.int64, 6, o)
     return {"hit_culled": hit_culled.astype(bool), "min": md, "unit": uv,
             "max_queue": int(obs[0]), "max_candidates": int(obs[1]), "max_unit_queue": int(obs[2]),
-            "groups": int(obs[3])}
+            "groups": int(obs[3]), "max_self_survivors": int(obs[4]), "self_overflows": int(obs[5])}
 
 
 def edge_configs(rng, lo, hi, n_edges: int, step: float) -> np.ndarray:
@@ -295,5 +295,6 @@ def test_simt_self_pairs_only(simt_exe, tmp_path, robot):
     q = np.concatenate([edge_configs(rng, sc.robot.lower, sc.robot.upper, 12, 0.1),
                         rng.uniform(sc.robot.lower, sc.robot.upper, size=(64, dim)).astype(np.float32)])
     hits = (0.02, 0.98) if robot == "arm" else (0.0, 1.0)  # the two arms overlap in most configurations
-    check_case(simt_exe, tmp_path, cc, dim, q, 0, 0.0, expect_hits=hits)
+    res, _ = check_case(simt_exe, tmp_path, cc, dim, q, 0, 0.0, expect_hits=hits)
+    print(robot, "survivors", res["max_self_survivors"], "overflows", res["self_overflows"])
     check_case(simt_exe, tmp_path, cc, dim, q, 1, 0.01, expect_hits=hits)
