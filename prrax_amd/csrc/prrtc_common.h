@@ -255,6 +255,13 @@ __device__ inline void stage_model(Model& M, char* lds, const ModelLayout& Lo, c
     }
     for (int k = tid; k < nwh; k += nt) WH[k] = make_halfspace(wh + 6 * k);
     __syncthreads();
+    // a serial chain of actuated revolute joints takes fk_lane_chain (same arithmetic, no branches)
+    int chain = nj > 0 ? 1 : 0;
+    for (int j = 0; j < nj; ++j) {
+        const JointRec& R = J[j];
+        if (R.type != kRot || R.mimic >= 0 || R.act < 0 || R.parent != j - 1) chain = 0;
+    }
+    M.chain = chain;
     // Self pairs: keep the valid ones (both indices in range, both spheres on a valid frame),
     // in their original order.
     int base = 0;

@@ -152,6 +152,7 @@ struct Model {
     const float4* whs;
     int n_joints, n_spheres, n_pairs, n_ws, n_wc, n_wb, n_wh;
     int enabled;
+    int chain;      // 1: a serial chain of revolute joints, joint j actuated, parent j - 1 (fk_lane_chain)
     int mode;       // 0 binary (also the coarse modes: same verdict), 1 margin
     float margin;
     // bounding sphere of each frame's robot spheres (groups in frame order)
@@ -282,7 +283,44 @@ struct LaneScratch {
     int nt;
 };
 
+// fk_lane for a serial chain of revolute joints (Model::chain): joint 0 has no parent, joint j's
+// parent is j - 1, every joint is actuated without mimic.  The arithmetic is fk_lane's for such
+// joints, operation for operation, without its per-joint branches on type, parent and mimic.
+__device__ __forceinline__ void fk_lane_chain(const Model& M, const LaneScratch& L) {
+    const int nt = L.nt, nj = M.n_joints;
+    Quat pq = {1.0f, 0.0f, 0.0f, 0.0f};
+    float ptx = 0.0f, pty = 0.0f, ptz = 0.0f;
+    JointRec Jn = M.joints[0];
+    for (int j = 0; j < nj; ++j) {
+        const JointRec J = Jn;
+        if (j + 1 < nj) Jn = M.joints[j + 1];
+        const float theta = L.q[J.act * nt];
+        const Quat P = {J.pw, J.px, J.py, J.pz};
+        Quat aq;
+        float atx, aty, atz;
+        if (j == 0) {
+            aq = P; atx = J.ptx; aty = J.pty; atz = J.ptz;
+        } else {
+            aq = qmul(pq, P);
+            const float3 t = mat_apply(quat_mat(pq), ptx, pty, ptz, J.ptx, J.pty, J.ptz);
+            atx = t.x; aty = t.y; atz = t.z;
+        }
+        float s, c;
+        sincos_pr(0.5f * (J.wn * theta), s, c);
+        const Quat E = {c, s * J.ax, s * J.ay, s * J.az};
+        const Quat oq = qmul(aq, E);
+        float* To = L.T + (j * 7) * nt;
+        To[0] = oq.w; To[nt] = oq.x; To[2 * nt] = oq.y; To[3 * nt] = oq.z;
+        To[4 * nt] = atx; To[5 * nt] = aty; To[6 * nt] = atz;
+        pq = oq; ptx = atx; pty = aty; ptz = atz;
+    }
+}
+
 __device__ __forceinline__ void fk_lane(const Model& M, const LaneScratch& L) {
+    if (M.chain) {
+        fk_lane_chain(M, L);
+        return;
+    }
     const int nt = L.nt;
     Quat pq = {1.0f, 0.0f, 0.0f, 0.0f};
     float ptx = 0.0f, pty = 0.0f, ptz = 0.0f;

@@ -535,8 +535,19 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                 qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qpt(list[c0 + c], d) : 0.0f;
             }
             __syncthreads();
-            nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
-                               t ? coh_b : coh_a);
+            // 7 dimensions: the 7-term instantiation (the 8-term one adds fma(0, 0, s) = s, the
+            // same sums at 8/7 of the arithmetic)
+            bool done7 = false;
+            if constexpr (MAXD == 8) {
+                if (dim == 7) {
+                    nn_tile<7, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
+                                    t ? coh_b : coh_a);
+                    done7 = true;
+                }
+            }
+            if (!done7)
+                nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
+                                   t ? coh_b : coh_a);
             if (tid == 0) {  // node bytes of this tile pass (no return value: the atomics do not wait)
                 global_add(&a.stats[16], 4ull * static_cast<unsigned long long>(dim) * static_cast<unsigned>(hi - lo));
                 global_add(&a.stats[17], 1ull);
