@@ -36,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes of one batched step of the 1024-query workload (plan_pass1 + plan_pass2): rocprofv3
 # FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) plus WRITE_SIZE (x 1.0), measured by the
 # profile runs named in "source", not by this run.  Reported only for the launch shape they ran.
-TRAFFIC = {"bytes_per_step": 13.28e9,
+TRAFFIC = {"bytes_per_step": 13.95e9,
            "source": "profiles/plan_batched_1024q_hbm_counters.md: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
-                     "of the same workload at this launch shape (337.4 ms step)",
+                     "of the timed step of the same workload at this launch shape (286.8 ms step)",
            "shape": {"num_new_samples": 512, "slots": 256, "threads": 512, "pass1_rounds": 24, "team_max": 512,
                      "fk_batch": 16}}
 
@@ -119,6 +119,34 @@ def time_singles(singles, label: str):
             log(f"{label}: {k + 1}/{len(singles)} single-query launches, {sum(times):.2f}s")
     solved = np.array([sp.bufs.host_summary()["solved"][0] == 1 for sp in singles])
     return np.array(times), solved
+
+
+def tail_leg(q: int, start, goals, world, max_nodes: int, kw: dict) -> dict:
+    """The longest query of the per-query leg alone, on a team of every work-group (team cap 256)
+    and on a team of 64 (its share while four long queries run at once in the batch): time, rounds
+    and the device-clock time per round of each phase (the planner's phase counters, 100 MHz)."""
+    import torch
+
+    from prrax_amd import prepare_batch
+
+    out = {"query": int(q), "world": {"spheres": int(len(world["world_spheres"])), "boxes": int(len(world["world_boxes"]))},
+           "max_nodes": int(max_nodes), "teams": {}}
+    for tm in (256, 64):
+        sp = prepare_batch(start[None], [goals], collision_context=[world], max_nodes=max_nodes, team_max=tm, **kw)
+        sp.enqueue()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sp.enqueue()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        st, h = sp.bufs.stats(), sp.bufs.host_summary()
+        r = max(int(h["iterations"][0]), 1)
+        out["teams"][str(tm)] = {"ms": round(1000 * dt, 2), "rounds": r, "us_per_round": round(1e6 * dt / r, 1),
+                                 "status": int(h["status"][0]), "tree_sizes": [int(v) for v in h["sizes"][0]],
+                                 "phase_us_per_round": {k: round(v / r / 100.0, 2) for k, v in st["pass2_ticks"].items()}}
+        del sp
+    log(f"tail query: {out}")
+    return out
 
 
 def validate_leg(results, worlds) -> dict:
@@ -652,6 +680,8 @@ def main():
               "longest_ms": round(1000 * float(times.max()), 1)}
         log(f"per-query: {pq}")
         del singles
+        qt = int(np.argmax(times))
+        pq["tail"] = tail_leg(qt, starts[qt], goals[qt], worlds[qt], nodes[qt], kw)
     fkcc = fkcc_leg(args.fkcc_configs) if args.fkcc_configs > 0 else None
     c2 = c2_leg(args.c2_queries, {k: v for k, v in kw.items() if k in ("max_iterations", "step_size", "granularity")}) \
         if args.c2_queries > 0 else None
@@ -659,7 +689,10 @@ def main():
     refine = refine_leg(args.refine_paths) if args.refine_paths > 0 else None
     dual = dual_arm_leg(args.dual_arm_queries) if args.dual_arm_queries > 0 else None
     cloud, cloud_setup = cloud_leg(args.cloud_points, args.cloud_queries) if args.cloud_points > 0 else (None, None)
-    achieved = stats["flops"] / (kernel_ms / 1000.0) / 1e12
+    # algorithmic FP32 work of the step: FK + collision tests performed, plus nearest-neighbour
+    # scoring (3 * dim flops per (query, node) pair: a subtraction and an FMA per dimension)
+    nn_flops = 3 * 7 * stats["nn_pairs"]
+    achieved = (stats["flops"] + nn_flops) / (kernel_ms / 1000.0) / 1e12
     shape_now = dict(stats.get("launch", {}), num_new_samples=kw["num_new_samples"])
     same_shape = args.queries == 1024 and all(shape_now.get(k) == v for k, v in TRAFFIC["shape"].items())
     nn_gbs = stats["nn_bytes"] / (kernel_ms / 1000.0) / 1e9
@@ -705,7 +738,8 @@ def main():
                      "traffic": TRAFFIC["bytes_per_step"] if same_shape else None,
                      "traffic_source": TRAFFIC["source"] if same_shape else
                      f"not measured for this launch shape (profiled: {TRAFFIC['shape']})",
-                     "flops_per_launch": stats["flops"], "nn_bytes_per_launch": stats["nn_bytes"],
+                     "flops_per_launch": stats["flops"] + nn_flops, "fkcc_flops_per_launch": stats["flops"],
+                     "nn_flops_per_launch": nn_flops, "nn_bytes_per_launch": stats["nn_bytes"],
                      "nn_achieved_gbs": round(nn_gbs, 2), "nn_frac_of_hbm": round(nn_gbs / PEAK_HBM_GBS, 5)},
         "device": dev_name,
     }

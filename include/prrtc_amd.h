@@ -154,7 +154,7 @@ size_t prrtc_planner_workspace_bytes(int32_t batch, int32_t dim, int32_t max_nod
  * the team pass (sample + NN, extend checks, commit, connect NN, connect walk, connect commit),
  * [16] algorithmic bytes of the nearest-neighbour scans (4 * dim per node per pass of a tile of
  * up to 16 queries), [17] such tile passes, [18] configurations per FK tile of a wave (8 or 16),
- * [19..31] zero.
+ * [19] (query, node) pairs those passes scored (3 * dim flops each), [20..31] zero.
  * No reference counterpart (the reference reports only kernel_time_ms). */
 size_t prrtc_planner_stats_offset(int32_t batch, int32_t dim, int32_t max_nodes, int32_t num_new_samples);
 

@@ -548,9 +548,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             if (!done7)
                 nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
                                    t ? coh_b : coh_a);
-            if (tid == 0) {  // node bytes of this tile pass (no return value: the atomics do not wait)
+            if (tid == 0) {  // node bytes, passes and (query, node) pairs (no return value: the atomics do not wait)
                 global_add(&a.stats[16], 4ull * static_cast<unsigned long long>(dim) * static_cast<unsigned>(hi - lo));
                 global_add(&a.stats[17], 1ull);
+                global_add(&a.stats[19], static_cast<unsigned long long>(nc) * static_cast<unsigned>(hi - lo));
             }
             if (tid < nc) gmin64(&keys[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
             __syncthreads();

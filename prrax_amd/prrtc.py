@@ -134,7 +134,7 @@ class PlanBuffers:
         pass."""
         raw = self.workspace[self.stats_offset:self.stats_offset + 256].cpu().numpy().view(np.uint64)
         out = {"configs": int(raw[0]), "rounds": int(raw[1]), "flops": int(raw[2]), "nn_bytes": int(raw[16]),
-               "nn_tile_passes": int(raw[17])}
+               "nn_tile_passes": int(raw[17]), "nn_pairs": int(raw[19])}
         shape = int(raw[3])
         out["launch"] = {"slots": shape & 0xFFFF, "threads": (shape >> 16) & 0xFFFF,
                          "pass1_rounds": (shape >> 32) & 0xFFFF, "team_max": (shape >> 48) & 0xFFFF,
