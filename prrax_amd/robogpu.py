@@ -175,14 +175,18 @@ class _HostTree:
         self.parents[0] = 0
         self.size = 1
 
-    def add(self, config: np.ndarray, parent: int) -> int:
-        if self.size >= self.configs.shape[0]:
-            return -1
-        i = self.size
-        self.configs[i] = config
-        self.parents[i] = parent
-        self.size += 1
-        return i
+    def room(self) -> int:
+        return self.configs.shape[0] - self.size
+
+    def add_many(self, configs: np.ndarray, parents: np.ndarray) -> np.ndarray:
+        """Appends rows in order until the tree is full; returns the indices of those that fit
+        (the reference adds one node at a time and stops at the first that does not fit)."""
+        n = min(configs.shape[0], self.configs.shape[0] - self.size)
+        i0 = self.size
+        self.configs[i0:i0 + n] = configs[:n]
+        self.parents[i0:i0 + n] = parents[:n]
+        self.size += n
+        return np.arange(i0, i0 + n, dtype=np.int64)
 
     def nearest(self, queries: np.ndarray, chunk: int = 8192) -> np.ndarray:
         """Lowest-index nearest node of each query.  Per (node, query) the squared distance is the
@@ -264,45 +268,42 @@ def prrtc_plan_robogpu(start_config, goal_configs, robot, robogpu_checker: Point
         near_idx = active.nearest(samples)
         q_near = active.configs[:active.size][near_idx]
         q_new = _steer(q_near, samples, step_size)
-        free = edges_free(q_near, q_new)
-        new_nodes, new_configs = [], []
-        for s in np.nonzero(free)[0]:
-            idx = active.add(q_new[s], int(near_idx[s]))
-            if idx < 0:
-                break
-            new_nodes.append(idx)
-            new_configs.append(q_new[s])
-        if not new_nodes:
+        fs = np.nonzero(edges_free(q_near, q_new))[0]
+        active_ids = active.add_many(q_new[fs], near_idx[fs])
+        if active_ids.size == 0:
             active, other = other, active
             continue
-        targets = np.stack(new_configs, axis=0)
-        active_ids = np.asarray(new_nodes, np.int32)
+        targets = q_new[fs[:active_ids.size]]
         other_idx = other.nearest(targets)
         live = np.ones(targets.shape[0], bool)
+        reach = step_size * 1e-3 + 1e-6
         for _ in range(connect_max_steps):
             if not live.any():
                 break
             li = np.nonzero(live)[0]
             q_from = other.configs[:other.size][other_idx[li]]
-            q_tgt = targets[li]
-            q_step = _steer(q_from, q_tgt, step_size)
+            q_step = _steer(q_from, targets[li], step_size)
             free_c = edges_free(q_from, q_step)
-            for k, s in enumerate(li):
-                if not free_c[k]:
-                    live[s] = False
-                    continue
-                idx = other.add(q_step[k], int(other_idx[s]))
-                if idx < 0:
-                    live[s] = False
-                    continue
-                other_idx[s] = idx
-                if np.linalg.norm(q_step[k] - targets[s]) <= step_size * 1e-3 + 1e-6:
-                    live[s] = False
-                    a_node, b_node = int(active_ids[s]), idx
-                    connection = (a_node, b_node) if active is tree_a else (b_node, a_node)
-                    solved = True
-                    break
-            if solved:
+            # The reference walks li in order: a blocked step ends its target, a free one is
+            # added (until the tree is full) and the first added step within reach of its
+            # target connects the trees, leaving the rest of li unvisited.
+            live[li[~free_c]] = False
+            ok = np.nonzero(free_c)[0]
+            fit = ok[:other.room()]
+            diff = q_step[fit] - targets[li[fit]]
+            hit = np.nonzero(np.sqrt(np.einsum("kd,kd->k", diff, diff)) <= reach)[0]
+            if hit.size:
+                fit = fit[:hit[0] + 1]
+            ids = other.add_many(q_step[fit], other_idx[li[fit]])
+            other_idx[li[fit]] = ids
+            if not hit.size:
+                live[li[ok[fit.size:]]] = False
+            else:
+                s = int(li[fit[-1]])
+                live[s] = False
+                a_node, b_node = int(active_ids[s]), int(ids[-1])
+                connection = (a_node, b_node) if active is tree_a else (b_node, a_node)
+                solved = True
                 break
         if solved:
             break
