@@ -114,6 +114,7 @@ struct EdgeArgs {
     CloudDev cloud;
     int has_cloud;
     int tile;                // waypoints per wave batch (FK one per lane), 1..kFB
+    int parts;               // waves per waypoint when tile == 1 (they split its cloud rows), else 1
     uint8_t* out_free;       // [n_edges], 1 on entry
     unsigned long long* flops;
 };
@@ -125,7 +126,10 @@ __global__ void edge_init_kernel(uint8_t* out_free, int n) {
 // Waves take batches of `tile` waypoints (FK one per lane), then check them one by one with the
 // whole wave: the world and self pairs (the planner's culled verdict), then the cloud.  An edge
 // already found colliding skips its remaining waypoints.  Small calls (a planner round's edges)
-// take tiles of fewer waypoints, so that they still spread over the GPU's waves.
+// take tiles of fewer waypoints, so that they still spread over the GPU's waves; the smallest
+// give each waypoint `parts` waves, which share its cloud rows (part 0 also tests the world).
+constexpr int kMaxParts = 16;
+
 __global__ __launch_bounds__(1024) void edge_check_kernel(EdgeArgs a) {
     extern __shared__ __align__(16) char lds[];
     int* scan = reinterpret_cast<int*>(lds + a.off_scan);
@@ -140,10 +144,12 @@ __global__ __launch_bounds__(1024) void edge_check_kernel(EdgeArgs a) {
     const int wpb = blockDim.x / kWave;
     const int total_waves = gridDim.x * wpb;
     const long long n = static_cast<long long>(a.n_edges) * a.G;
-    const int tile = a.tile;
-    const long long nb = (n + tile - 1) / tile;
+    const int tile = a.tile, parts = a.parts;
+    const long long nb = (n + tile - 1) / tile * parts;
     unsigned long long flops = 0;
-    for (long long b = static_cast<long long>(blockIdx.x) * wpb + threadIdx.x / kWave; b < nb; b += total_waves) {
+    for (long long bp = static_cast<long long>(blockIdx.x) * wpb + threadIdx.x / kWave; bp < nb; bp += total_waves) {
+        const long long b = bp / parts;
+        const int part = static_cast<int>(bp - b * parts);
         const long long i = b * tile + lane;
         if (lane < tile && i < n) {
             for (int d = 0; d < a.dim; ++d) L.q[d * kFBP] = a.waypoints[i * a.dim + d];
@@ -157,8 +163,8 @@ __global__ __launch_bounds__(1024) void edge_check_kernel(EdgeArgs a) {
             const int open = __builtin_amdgcn_readfirstlane(
                 static_cast<int>(*reinterpret_cast<volatile const uint8_t*>(a.out_free + e)));
             if (open == 0) continue;
-            bool h = wave_in_collision(M, W.T + c, kFBP, flops);
-            if (!h && a.has_cloud && M.n_joints > 0) h = wave_cloud_hit(M, W.T + c, kFBP, a.cloud, flops);
+            bool h = part == 0 && wave_in_collision(M, W.T + c, kFBP, flops);
+            if (!h && a.has_cloud && M.n_joints > 0) h = wave_cloud_hit(M, W.T + c, kFBP, a.cloud, flops, part, parts);
             if (h && lane == 0) a.out_free[e] = 0;
         }
         wave_sync();
@@ -324,7 +330,11 @@ int prrtc_check_edges(prrtc_stream_t stream, const float* waypoints, int32_t n_e
     int tile = kFB;
     while (tile > 1 && (nwp + tile - 1) / tile < 16ll * di.cus) tile /= 2;
     a.tile = tile;
-    const long long batches = (nwp + tile - 1) / tile;
+    int parts = 1;
+    if (tile == 1 && a.has_cloud)
+        while (parts < kMaxParts && nwp * parts * 2 <= 16ll * di.cus) parts *= 2;
+    a.parts = parts;
+    const long long batches = (nwp + tile - 1) / tile * parts;
     long long grid = (launch && launch->slots > 0) ? launch->slots : (batches + nt / kWave - 1) / (nt / kWave);
     grid = std::max(1ll, std::min(grid, static_cast<long long>(8 * di.cus)));
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_check_kernel),

@@ -8,9 +8,11 @@
 // Input records (int64 count, then elements): the robot and world records of simt_check, then
 // params i32 [dim, mode, n_configs, nx, ny, nz], floats f32 [margin, lo_x, lo_y, lo_z, cell,
 // r_env], points f32 [n_points * 3], configs f32 [n_configs * dim].
-// Output: world verdict u8[n], cloud verdict u8[n] (computed for every configuration).
+// Output: world verdict u8[n], cloud verdict u8[n] (computed for every configuration).  Exit
+// status 3 when the cloud verdict split over three waves' shares of the rows differs from it.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <fstream>
 #include <thread>
 #include <vector>
@@ -121,6 +123,8 @@ int main(int argc, char** argv) {
     std::vector<char> lds(ml.end);
     std::vector<int> scan(17);
     std::vector<uint8_t> hit_world(n), hit_cloud(n);
+    constexpr int kParts = 3;
+    bool split_mismatch = false;
     blockDim.x = simt::kLanes;
     gridDim.x = 1;
     auto body = [&](int lane) {
@@ -145,6 +149,11 @@ int main(int argc, char** argv) {
             for (int c = 0; c < nc; ++c) {
                 const bool hw = wave_in_collision(M, W.T + c, kFBP, flops);
                 const bool hc = (M.n_joints > 0 && np > 0) ? wave_cloud_hit(M, W.T + c, kFBP, C, flops) : false;
+                // the same configuration split over kParts waves (run here one after another)
+                bool hs = false;
+                for (int part = 0; part < kParts && M.n_joints > 0 && np > 0; ++part)
+                    hs = wave_cloud_hit(M, W.T + c, kFBP, C, flops, part, kParts) || hs;
+                if (lane == 0 && hs != hc) split_mismatch = true;
                 if (lane == 0) {
                     hit_world[b0 + c] = hw ? 1 : 0;
                     hit_cloud[b0 + c] = hc ? 1 : 0;
@@ -156,6 +165,10 @@ int main(int argc, char** argv) {
     std::vector<std::thread> lanes;
     for (int l = 0; l < simt::kLanes; ++l) lanes.emplace_back(body, l);
     for (auto& t : lanes) t.join();
+    if (split_mismatch) {
+        std::fprintf(stderr, "cloud verdict split over %d parts differs from the whole\n", kParts);
+        return 3;
+    }
     std::ofstream out(argv[2], std::ios::binary);
     out.write(reinterpret_cast<const char*>(hit_world.data()), n);
     out.write(reinterpret_cast<const char*>(hit_cloud.data()), n);
