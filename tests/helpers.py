@@ -20,6 +20,18 @@ def query(scene: scenes.Scene, config: int, qi: int, n_goals: int = 1):
     return sg[0], sg[1:]
 
 
+def plan_digests(tree_a, tree_b, parents_a, parents_b, path) -> tuple:
+    """SHA-256 digests (32 bytes each) of a plan's trees [dim, size] (FP32 bits), parents (int32)
+    and path [L, dim] (FP32 bits; the digest of no bytes when there is none)."""
+    import hashlib
+
+    def h(a, dt):
+        return hashlib.sha256(b"" if a is None else np.ascontiguousarray(np.asarray(a), dt).tobytes()).digest()
+
+    return (h(tree_a, np.float32), h(tree_b, np.float32), h(parents_a, np.int32), h(parents_b, np.int32),
+            h(path, np.float32))
+
+
 def compare_plan(gpu, ora: dict, tag: str = "") -> None:
     """GPU PRRTCResult vs oracle plan dict, bit for bit."""
     assert gpu.status == ora["status"], f"{tag} status {gpu.status} != {ora['status']}"
