@@ -336,7 +336,7 @@ int prrtc_check_edges(prrtc_stream_t stream, const float* waypoints, int32_t n_e
     a.parts = parts;
     const long long batches = (nwp + tile - 1) / tile * parts;
     long long grid = (launch && launch->slots > 0) ? launch->slots : (batches + nt / kWave - 1) / (nt / kWave);
-    grid = std::max(1ll, std::min(grid, static_cast<long long>(8 * di.cus)));
+    grid = std::max(1ll, std::min(grid, static_cast<long long>(16 * di.cus)));
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&edge_check_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
