@@ -211,11 +211,12 @@ int prrtc_check_configs(prrtc_stream_t stream, const float* configs, int32_t n, 
     a.flops = reinterpret_cast<unsigned long long*>(work_flops);
     const int batches = (n + kFB - 1) / kFB;
     int grid = (launch && launch->slots > 0) ? launch->slots : (batches + nt / kWave - 1) / (nt / kWave);
-    // up to 32 work-groups per CU, more than stay resident, so the hardware hands the last ones
-    // to whichever CUs free up first (binary verdicts of 1e6 configurations of the large scene:
-    // 8 per CU 1.30 ms, 32 per CU 1.21 ms, 64 per CU 1.32 ms; each work-group first stages the
-    // robot and world into LDS)
-    grid = std::max(1, std::min(grid, 32 * di.cus));
+    // up to 16 work-groups per CU, more than stay resident, so the hardware hands the last ones
+    // to whichever CUs free up first; each work-group first stages the robot and world into LDS.
+    // Measured with 8 / 16 / 32 per CU: binary verdicts of 1e6 configurations of the large scene
+    // 1.30 / 1.22 / 1.21 ms, the 512-path refinement (100 calls of ~5e5 configurations) 73.0 /
+    // 69.9 / 78.6 ms.
+    grid = std::max(1, std::min(grid, 16 * di.cus));
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&check_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return fail_with(PRRTC_E_HIP, hipGetErrorString(e));
