@@ -49,7 +49,7 @@ namespace prrtc {
 enum { XG_KEY = 0, XG_OKEY = 2, XG_FAIL = 4, XG_VAMP, XG_ECLAIM, XG_CLAIM, XG_CFAIL, XG_PAD, XG_COUNT };
 static_assert(XG_COUNT % 2 == 0, "keys of every query must stay 8-byte aligned");
 // Per-query round state ([B][QS_COUNT] ints) for the pass-1 -> pass-2 hand-over.
-enum { QS_ROUND, QS_SIZE_A, QS_SIZE_B, QS_TIME, QS_COUNT = 8 };
+enum { QS_ROUND, QS_SIZE_A, QS_SIZE_B, QS_TIME, QS_STOP, QS_COUNT = 8 };
 // Control words ([kCtl] ints).  CTL_NC + i counts continue list i (three lists rotate over the
 // epochs of pass 2: one read, one filled, one cleared).
 enum { CTL_QUEUE, CTL_ERROR, CTL_NC, kCtl = 16 };
@@ -123,7 +123,7 @@ __host__ __device__ inline int hash_size_for(int S) {
 }
 
 enum { SC_QUERY = 0, SC_SIZE_A, SC_SIZE_B, SC_STATUS, SC_EVENT, SC_OVERFLOW, SC_CONN0, SC_CONN1, SC_CONN2, SC_ERR,
-       SC_DRAIN, SC_COUNT = 16 };
+       SC_DRAIN, SC_STOP, SC_COUNT = 16 };
 
 // Pass 1 gives a query at least this many rounds; after that, once no query is left to start,
 // the query moves on to pass 2 (teams) instead of spending the rest of its budget on one
@@ -427,7 +427,8 @@ struct QueryView {
 
 template <int MAXD, bool COH>
 __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team& team, int r_begin, int r_budget,
-                          unsigned long long t0, int pass, int* push_cnt, int* push_list) {
+                          unsigned long long t0, int pass, int* push_cnt, int* push_list,
+                          unsigned long long deadline = 0ull) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & (kWave - 1), wid = tid / kWave, nwaves = nt / kWave;
     const int dim = a.dim, S = a.S, G = a.G, ncap = a.ncap, N = V.N, W = team.W, m = team.m;
@@ -563,7 +564,7 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         nn_phase([&](int e) { return newi[e] >= 0 ? 1 - side[e] : -1; },
                  [&](int e, int d) { return cand[e * dim + d]; }, V.g_okey, sz0, sz1);
     };
-    const int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
+    int r_end = (r_budget >= 0) ? r_begin + r_budget : 0x7fffffff;  // r_budget < 0: no budget
     bool sampled = false;  // cand holds this round's samples (drawn during the previous B8)
     int r = r_begin;
     for (;; ++r) {
@@ -1002,7 +1003,10 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             }
         }
         // B8: connect nodes are in global memory.  Between arriving and waiting, the work-group
-        // draws the next round's samples (LDS only; phase 6 no longer reads cand).
+        // draws the next round's samples (LDS only; phase 6 no longer reads cand).  With a
+        // deadline, member 0 decides before it arrives whether this is the epoch's last round;
+        // every member reads that decision after the barrier.
+        if (deadline != 0ull && m == 0 && tid == 0) gstore(&V.qs[QS_STOP], wall_clock64() >= deadline ? 1 : 0);
         team_arrive(team);
         halton_round(lds + a.off_sampler, r + 1);
         __syncthreads();
@@ -1014,6 +1018,11 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
         if (W > 1) {
             if (tid == 0) sc[SC_ERR] = gload(team.error);
             __syncthreads();
+        }
+        if (deadline != 0ull) {
+            if (tid == 0) sc[SC_STOP] = gload(&V.qs[QS_STOP]);
+            __syncthreads();
+            if (sc[SC_STOP]) r_end = r + 1;  // the epoch's time is up: save the state after this round
         }
         if (sc[SC_STATUS] != 0 || sc[SC_ERR]) break;
     }
@@ -1191,15 +1200,79 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
     }
 }
 
-// Pass 2: teams of W work-groups resume the queries left open by pass 1, in epochs.  In each
-// epoch the open queries are dealt to teams (W = slots / open queries, capped)
-// and run for at most 4 << epoch rounds (capped at `epoch_rounds`); the queries still open then
-// form the next epoch's list, so the teams grow as queries finish.  A grid barrier separates
-// the epochs.
+// Team sizes of a tail of open queries (2 .. kTailMax of them, at least two work-groups each):
+// wave 0 of every work-group computes the same layout from the same saved round states, so the
+// members of a team agree on it without another grid barrier.  A query's remaining rounds are
+// estimated from its trees' growth so far (a tree that fills ends the query): a tree of n nodes
+// after r rounds reaches max_nodes N after about r (N / n)^(4/3) rounds (the C3 queries that
+// fill their trees grow as r^0.75 after their first hundred rounds: q455 holds 6.6k, 19.3k and
+// 56.0k nodes after 100, 400 and 1600 rounds), at most the rounds left to max_iterations.  A team's round time is modelled as proportional to 1 + kTeamLatency / W
+// (a latency-bound part and a part that spreads over the members; the longest C3 query takes
+// 169, 123 and 116 us per round on teams of 64, 128 and 256).  Work-groups go, a few at a time,
+// to the query with the latest estimated finish (remaining rounds x round time), lowest list
+// index on ties, up to the team cap.  start[k] .. start[k + 1] are the work-groups of team k (the
+// query at position k of the list); work-groups from start[ncont] on have no team this epoch.
+constexpr int kTailMax = 32;
+constexpr float kTeamLatency = 43.0f;
+constexpr float kEpochRoundUs = 150.0f;  // a tail round's time on a share of the GPU (C3: 110-170 us)
+
+__device__ void tail_teams(const PlanArgs& a, int cur, int ncont, int* start) {
+    const int lane = threadIdx.x & (kWave - 1);
+    float rem = 0.0f;
+    if (lane < ncont) {
+        const int q = a.cont[cur * a.B + lane];
+        const int* qs = a.qstate + static_cast<size_t>(q) * QS_COUNT;
+        const int ng = a.goal_counts ? max(1, min(a.goal_counts[q], a.gmax)) : a.gmax;
+        const int N = a.max_nodes_q ? min(max(a.max_nodes_q[q], ng), a.ncap) : a.ncap;
+        const int done = qs[QS_ROUND] - 1;
+        float rm = static_cast<float>(max(a.max_iterations - done, 1));
+        if (done > 0) {
+            const int size[2] = {qs[QS_SIZE_A], qs[QS_SIZE_B]};
+            for (int t = 0; t < 2; ++t) {
+                const float fill = static_cast<float>(N) / static_cast<float>(max(size[t], 1));
+                rm = fminf(rm, static_cast<float>(done) * (exp2f(1.3333333f * log2f(fmaxf(fill, 1.0f))) - 1.0f));
+            }
+        }
+        rem = fmaxf(rm, 1.0f);
+    }
+    int W = lane < ncont ? 1 : 0;
+    for (int left = a.slots - ncont; left > 0;) {
+        const int give = max(1, left / (2 * ncont));  // a few dozen steps, finer as they run out
+        const float fin = (lane < ncont && W < a.wmax) ? rem * (1.0f + kTeamLatency / static_cast<float>(W)) : -1.0f;
+        // wave-wide maximum of (estimated finish, lowest lane): finish times are positive, so
+        // their bits order as unsigned integers
+        unsigned long long key = fin < 0.0f ? 0ull
+                                            : (static_cast<unsigned long long>(__float_as_uint(fin)) << 32) |
+                                                  static_cast<unsigned>(kWave - 1 - lane);
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const unsigned long long other = __shfl_xor(key, o, kWave);
+            key = other > key ? other : key;
+        }
+        if (key == 0ull) break;  // every team at the cap
+        if (lane == kWave - 1 - static_cast<int>(key & 0xffffffffull)) W = min(W + give, max(a.wmax, 1));
+        left -= give;
+    }
+    int incl = W;  // inclusive prefix sum over the lanes
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 0) start[0] = 0;
+    if (lane < ncont) start[lane + 1] = incl;
+}
+
+// Pass 2: teams of work-groups resume the queries left open by pass 1, in epochs.  In each
+// epoch the open queries are dealt to teams -- a tail of at most kTailMax queries to teams sized
+// by tail_teams, more queries to equal teams of W = slots / open queries (capped) -- and run for
+// at most 4 << epoch rounds (capped at `epoch_rounds`); the queries still open then form the next
+// epoch's list, so the teams grow as queries finish.  A grid barrier separates the epochs.
 template <int MAXD>
 __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_ncont;
+    __shared__ int s_start[kTailMax + 1];
     const int b = blockIdx.x;
     Team grid = make_team(a, a.slots, a.slots, b);
     sampler_init(lds + a.off_sampler, a.min_vals, a.max_vals, a.dim);  // batch-wide; synced below
@@ -1211,19 +1284,35 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
         if (ncont == 0) return;  // every work-group reads the same count after the grid barrier
         // list (ep + 2) % 3 was read in the previous epoch and is filled in the next one
         if (b == 0 && threadIdx.x == 0) gstore(&a.ctl[CTL_NC + (ep + 2) % 3], 0);
-        const int W = max(1, min(a.wmax, a.slots / ncont));  // every resident work-group has a team
-        int team_id, member;
-        if (a.slots % (8 * W) == 0) {  // keep a team on one XCD (work-groups are dealt round-robin)
-            team_id = (b % 8) + 8 * ((b / 8) / W);
-            member = (b / 8) % W;
+        int W, team_id, member, nteams;
+        const bool tail = ncont >= 2 && ncont <= kTailMax && a.slots >= 2 * ncont;
+        if (tail) {
+            if (threadIdx.x < kWave) tail_teams(a, cur, ncont, s_start);
+            __syncthreads();
+            team_id = ncont;  // no team
+            for (int k = 0; k < ncont; ++k)
+                if (b >= s_start[k] && b < s_start[k + 1]) team_id = k;
+            W = (team_id < ncont) ? s_start[team_id + 1] - s_start[team_id] : 1;
+            member = (team_id < ncont) ? b - s_start[team_id] : 0;
+            nteams = ncont;
         } else {
-            team_id = b / W;
-            member = b % W;
+            W = max(1, min(a.wmax, a.slots / ncont));  // every resident work-group has a team
+            if (a.slots % (8 * W) == 0) {  // keep a team on one XCD (work-groups are dealt round-robin)
+                team_id = (b % 8) + 8 * ((b / 8) / W);
+                member = (b / 8) % W;
+            } else {
+                team_id = b / W;
+                member = b % W;
+            }
+            nteams = a.slots / W;
         }
-        const int nteams = a.slots / W;
         // Epoch budgets grow 4, 8, 16, ... up to epoch_rounds: early epochs end soon, so the
         // teams re-form (and grow) while most queries of a batch finish.
         const int budget = (a.epoch_rounds > 0) ? min(a.epoch_rounds, 4 << min(ep, 20)) : -1;
+        // Tail teams run at different speeds, so their epochs end at a time, not a round count:
+        // about `budget` rounds of kEpochRoundUs each, and at most 4 x budget rounds.
+        const unsigned long long span = (tail && budget > 0)
+            ? static_cast<unsigned long long>(kEpochRoundUs * static_cast<float>(budget) * a.wall_khz * 1e-3f) : 0ull;
         if (team_id < nteams && team_id < ncont) {
             Team team = make_team(a, team_id, W, member);
             if (threadIdx.x == 0)  // the words are at rest: every barrier on them ended before the grid barrier
@@ -1232,8 +1321,8 @@ __global__ __launch_bounds__(512) void plan_pass2(PlanArgs a) {
             for (int k = team_id; k < ncont; k += nteams) {
                 const unsigned long long t0 = wall_clock64();
                 const QueryView V = query_view(a, a.cont[cur * a.B + k]);
-                run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], budget, t0, 2,
-                                a.ctl + CTL_NC + nxt, a.cont + nxt * a.B);
+                run_query<MAXD, true>(a, V, lds, team, V.qs[QS_ROUND], span ? 4 * budget : budget, t0, 2,
+                                a.ctl + CTL_NC + nxt, a.cont + nxt * a.B, span ? t0 + span : 0ull);
                 // members leave the query together before the next one; the team's last barrier
                 // resets its counters for the next epoch's team shapes
                 team_barrier(team, k + nteams >= ncont);
