@@ -1204,9 +1204,11 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
 // wave 0 of every work-group computes the same layout from the same saved round states, so the
 // members of a team agree on it without another grid barrier.  A query's remaining rounds are
 // estimated from its trees' growth so far (a tree that fills ends the query): a tree of n nodes
-// after r rounds reaches max_nodes N after about r (N / n)^(4/3) rounds (the C3 queries that
-// fill their trees grow as r^0.75 after their first hundred rounds: q455 holds 6.6k, 19.3k and
-// 56.0k nodes after 100, 400 and 1600 rounds), at most the rounds left to max_iterations.  A team's round time is modelled as proportional to 1 + kTeamLatency / W
+// after r rounds reaches max_nodes N after about r (N / n)^kGrowthExp rounds, at most the rounds
+// left to max_iterations.  (The C3 queries that fill their trees grow as r^0.75 after their first
+// hundred rounds, faster before: q455 holds 1.8k, 6.6k, 19.3k and 56.0k nodes after 25, 100, 400
+// and 1600 rounds.  Exponents 1.0, 1.15, 1.33 and 1.5 gave C3 steps of 255.2, 255.3, 258.2 and
+// 257.8 ms.)  A team's round time is modelled as proportional to 1 + kTeamLatency / W
 // (a latency-bound part and a part that spreads over the members; the longest C3 query takes
 // 169, 123 and 116 us per round on teams of 64, 128 and 256).  Work-groups go, a few at a time,
 // to the query with the latest estimated finish (remaining rounds x round time), lowest list
@@ -1214,7 +1216,10 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
 // query at position k of the list); work-groups from start[ncont] on have no team this epoch.
 constexpr int kTailMax = 32;
 constexpr float kTeamLatency = 43.0f;
-constexpr float kEpochRoundUs = 150.0f;  // a tail round's time on a share of the GPU (C3: 110-170 us)
+constexpr float kGrowthExp = 1.15f;
+// Epoch time per round of budget.  C3 steps (with kGrowthExp 1.15): 150 us 255.3 ms, 200 us
+// 256.1, 250 us 254.3, 300 us 254.0; 600 us (epochs of up to 38 ms) 290.3.
+constexpr float kEpochRoundUs = 250.0f;
 
 __device__ void tail_teams(const PlanArgs& a, int cur, int ncont, int* start) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -1230,7 +1235,7 @@ __device__ void tail_teams(const PlanArgs& a, int cur, int ncont, int* start) {
             const int size[2] = {qs[QS_SIZE_A], qs[QS_SIZE_B]};
             for (int t = 0; t < 2; ++t) {
                 const float fill = static_cast<float>(N) / static_cast<float>(max(size[t], 1));
-                rm = fminf(rm, static_cast<float>(done) * (exp2f(1.3333333f * log2f(fmaxf(fill, 1.0f))) - 1.0f));
+                rm = fminf(rm, static_cast<float>(done) * (exp2f(kGrowthExp * log2f(fmaxf(fill, 1.0f))) - 1.0f));
             }
         }
         rem = fmaxf(rm, 1.0f);
