@@ -108,8 +108,8 @@ def test_context_invariants():
 
 def test_default_queue_order_longest_first():
     """The default queue order of the one-work-group pass: a permutation, by start-to-nearest-
-    goal distance times obstacle load, descending; padded goal slots (beyond a query's goal
-    count) are ignored."""
+    goal distance times the square root of the obstacle load, descending; padded goal slots
+    (beyond a query's goal count) are ignored."""
     from prrax_amd.prrtc import _default_order
     starts = np.zeros((4, 2), np.float32)
     goals = np.zeros((4, 2, 2), np.float32)
@@ -128,4 +128,4 @@ def test_default_queue_order_longest_first():
         return {"world_spheres": np.zeros((ns, 4)), "world_boxes": np.zeros((nb, 15)),
                 "world_capsules": np.zeros((nc, 7))}
     o = _default_order(starts, goals, gcounts, [w(10), w(0), w(0, 1), w(0, 0, 1)])
-    assert o.tolist() == [0, 2, 1, 3]  # keys 11, 10 (2 x (1 + 4)), 3, 1.5
+    assert o.tolist() == [2, 0, 1, 3]  # keys 2 sqrt(5) = 4.47, sqrt(11) = 3.32, 3, 0.5 sqrt(3) = 0.87
