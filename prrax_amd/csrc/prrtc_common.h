@@ -413,6 +413,9 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
         }
         for (int i = begin + tid; i < end; i += 2 * nt) {
             asm volatile("" : "+v"(qoff));
+            // (this copy also frees ra / rb for the next loads: without it -- dim folded to 7 in
+            // the 7-term instantiation -- the compiler waited for the prefetch at the top of the
+            // loop, and the sample NN of a 64-member team took 76 instead of 53 us per round)
             float xa[MAXD], xb[MAXD];
 #pragma unroll
             for (int d = 0; d < MAXD; ++d) {
