@@ -1200,7 +1200,7 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
     }
 }
 
-// Team sizes of a tail of open queries (2 .. kTailMax of them, at least two work-groups each):
+// Team sizes of a tail of open queries (2 .. kTailMax of them, at least two work-groups per query):
 // wave 0 of every work-group computes the same layout from the same saved round states, so the
 // members of a team agree on it without another grid barrier.  A query's remaining rounds are
 // estimated from its trees' growth so far (a tree that fills ends the query): a tree of n nodes
@@ -1208,7 +1208,7 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
 // left to max_iterations.  (The C3 queries that fill their trees grow as r^0.75 after their first
 // hundred rounds, faster before: q455 holds 1.8k, 6.6k, 19.3k and 56.0k nodes after 25, 100, 400
 // and 1600 rounds.  Exponents 1.0, 1.15, 1.33 and 1.5 gave C3 steps of 255.2, 255.3, 258.2 and
-// 257.8 ms.)  A team's round time is modelled as proportional to 1 + kTeamLatency / W
+// 257.8 ms, at 150 us of epoch per round.)  A team's round time is modelled as proportional to 1 + kTeamLatency / W
 // (a latency-bound part and a part that spreads over the members; the longest C3 query takes
 // 169, 123 and 116 us per round on teams of 64, 128 and 256).  Work-groups go, a few at a time,
 // to the query with the latest estimated finish (remaining rounds x round time), lowest list
