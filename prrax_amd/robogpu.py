@@ -275,26 +275,51 @@ def prrtc_plan_robogpu(start_config, goal_configs, robot, robogpu_checker: Point
             continue
         targets = q_new[fs[:active_ids.size]]
         other_idx = other.nearest(targets)
-        live = np.ones(targets.shape[0], bool)
+        n = targets.shape[0]
         reach = step_size * 1e-3 + 1e-6
-        for _ in range(connect_max_steps):
+        # Every target's walk on the host first: a step that is taken starts where the previous
+        # one ended, so the positions do not depend on the verdicts.  A walk has at most
+        # connect_max_steps steps and ends at the first step within reach of its target.
+        pos = [other.configs[:other.size][other_idx]]
+        reach_step = np.zeros(n, np.int64)  # step that comes within reach (0: none so far)
+        for j in range(1, connect_max_steps + 1):
+            pos.append(_steer(pos[-1], targets, step_size))
+            diff = pos[j] - targets
+            now = (reach_step == 0) & (np.sqrt(np.einsum("kd,kd->k", diff, diff)) <= reach)
+            reach_step[now] = j
+            if (reach_step > 0).all():
+                break
+        steps = len(pos) - 1
+        n_steps = np.where(reach_step > 0, reach_step, steps)
+        # The walks' steps are checked in blocks of 2, 4, 8, ... steps of the walks still live,
+        # one device call per block, and replayed in the reference's order.
+        P = np.stack(pos, axis=1)  # [n, steps + 1, dim]
+        free_grid = np.zeros((n, steps), bool)
+        live = np.ones(n, bool)
+        checked, block = 0, 2
+        for j in range(1, steps + 1):
             if not live.any():
                 break
+            if j > checked:
+                checked = min(steps, checked + block)
+                block *= 2
+                lv = np.nonzero(live)[0]
+                grid = (np.arange(j, checked + 1)[None, :] <= n_steps[lv, None])
+                tt, jj = np.nonzero(grid)
+                tt, jj = lv[tt], jj + (j - 1)
+                free_grid[tt, jj] = edges_free(P[tt, jj], P[tt, jj + 1])
             li = np.nonzero(live)[0]
-            q_from = other.configs[:other.size][other_idx[li]]
-            q_step = _steer(q_from, targets[li], step_size)
-            free_c = edges_free(q_from, q_step)
+            free_c = free_grid[li, j - 1]
             # The reference walks li in order: a blocked step ends its target, a free one is
             # added (until the tree is full) and the first added step within reach of its
             # target connects the trees, leaving the rest of li unvisited.
             live[li[~free_c]] = False
             ok = np.nonzero(free_c)[0]
             fit = ok[:other.room()]
-            diff = q_step[fit] - targets[li[fit]]
-            hit = np.nonzero(np.sqrt(np.einsum("kd,kd->k", diff, diff)) <= reach)[0]
+            hit = np.nonzero(reach_step[li[fit]] == j)[0]
             if hit.size:
                 fit = fit[:hit[0] + 1]
-            ids = other.add_many(q_step[fit], other_idx[li[fit]])
+            ids = other.add_many(pos[j][li[fit]], other_idx[li[fit]])
             other_idx[li[fit]] = ids
             if not hit.size:
                 live[li[ok[fit.size:]]] = False
