@@ -78,6 +78,61 @@ def test_host_loop_checker_matches_oracle_and_plans():
     assert validate.validate_paths(c1, [r["path"]], resolution=15)[0]
 
 
+class _OracleEdges:
+    """A CPU stand-in for PointCloudChecker.edges_free: the oracle's world verdict of every
+    waypoint, 1 where a whole edge is free (the loop's host logic runs unchanged)."""
+
+    def __init__(self, cc, dim):
+        self.chk, self.dim, self.calls = host_rrtc.BatchChecker(cc, dim), dim, 0
+
+    def edges_free(self, wp, world_geom=None, point_cloud=None, r_env=0.02):
+        import torch
+
+        self.calls += 1
+        wp = np.asarray(wp, np.float32)
+        return torch.from_numpy(self.chk.free(wp.reshape(-1, self.dim)).reshape(wp.shape[0], -1).all(1).astype(np.uint8))
+
+
+class _NoEvent:
+    def __init__(self, **_):
+        pass
+
+    def record(self):
+        pass
+
+    def elapsed_time(self, other):
+        return 0.0
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(max_nodes=120), dict(connect_max_steps=3), dict(num_new_samples=8)])
+def test_plan_robogpu_host_loop_equals_the_reference_loop(monkeypatch, kw):
+    """prrtc_plan_robogpu's host logic (walk positions on the host, connect steps checked in
+    blocks and replayed with array operations) equals oracle/host_rrtc.py's one-target-at-a-time
+    restatement of the reference loop with the same checker: iterations, tree sizes, trees and
+    path; also when the trees fill (max_nodes 120: query 0 fills both and never connects) and
+    with short walks."""
+    import torch
+
+    from prrax_amd.robogpu import prrtc_plan_robogpu
+
+    monkeypatch.setattr(torch.cuda, "Event", _NoEvent)
+    sc = scenes.scene_c1()
+    cc, robot = sc.context(), sc.robot
+    for qi in range(3):
+        start, goals = query(sc, 1, qi)
+        args = dict(max_iterations=60, step_size=0.5, num_new_samples=32, granularity=16, max_nodes=4000,
+                    connect_max_steps=32, seed=qi)
+        args.update(kw)
+        chk = _OracleEdges(cc, 7)
+        res = prrtc_plan_robogpu(start, goals, robot, chk, **args)
+        ref = host_rrtc.plan(host_rrtc.BatchChecker(cc, 7), start, goals, min_vals=robot.lower, max_vals=robot.upper,
+                             **args)
+        assert res.solved == ref["solved"] and res.iterations == ref["iterations"], (qi, kw)
+        assert (res.tree_a_size, res.tree_b_size) == ref["sizes"], (qi, kw)
+        if res.solved:
+            np.testing.assert_array_equal(res.path, ref["path"])
+
+
 def _c_layout() -> dict:
     src = r"""
     #include <stddef.h>
