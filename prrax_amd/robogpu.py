@@ -277,37 +277,41 @@ def prrtc_plan_robogpu(start_config, goal_configs, robot, robogpu_checker: Point
         other_idx = other.nearest(targets)
         n = targets.shape[0]
         reach = step_size * 1e-3 + 1e-6
-        # Every target's walk on the host first: a step that is taken starts where the previous
-        # one ended, so the positions do not depend on the verdicts.  A walk has at most
-        # connect_max_steps steps and ends at the first step within reach of its target.
+        # The walks' positions on the host: a step that is taken starts where the previous one
+        # ended, so they do not depend on the verdicts.  A walk has at most connect_max_steps
+        # steps and ends at the first step within reach of its target.  Its steps are checked
+        # in blocks of 2, 4, 8, ... steps of the walks still live, one device call per block
+        # (positions computed as far as the block needs), and replayed in the reference's order.
         pos = [other.configs[:other.size][other_idx]]
         reach_step = np.zeros(n, np.int64)  # step that comes within reach (0: none so far)
-        for j in range(1, connect_max_steps + 1):
-            pos.append(_steer(pos[-1], targets, step_size))
-            diff = pos[j] - targets
-            now = (reach_step == 0) & (np.sqrt(np.einsum("kd,kd->k", diff, diff)) <= reach)
-            reach_step[now] = j
-            if (reach_step > 0).all():
-                break
-        steps = len(pos) - 1
-        n_steps = np.where(reach_step > 0, reach_step, steps)
-        # The walks' steps are checked in blocks of 2, 4, 8, ... steps of the walks still live,
-        # one device call per block, and replayed in the reference's order.
-        P = np.stack(pos, axis=1)  # [n, steps + 1, dim]
-        free_grid = np.zeros((n, steps), bool)
+
+        def walk_to(upto: int) -> None:
+            while len(pos) <= upto and not (reach_step > 0).all():
+                j = len(pos)
+                pos.append(_steer(pos[-1], targets, step_size))
+                diff = pos[j] - targets
+                now = (reach_step == 0) & (np.sqrt(np.einsum("kd,kd->k", diff, diff)) <= reach)
+                reach_step[now] = j
+
+        free_grid = np.zeros((n, connect_max_steps), bool)
         live = np.ones(n, bool)
         checked, block = 0, 2
-        for j in range(1, steps + 1):
+        for j in range(1, connect_max_steps + 1):
             if not live.any():
                 break
             if j > checked:
-                checked = min(steps, checked + block)
+                walk_to(min(connect_max_steps, checked + block))
+                checked = len(pos) - 1
                 block *= 2
+                if j > checked:
+                    break  # every walk reached its target at an earlier step
                 lv = np.nonzero(live)[0]
-                grid = (np.arange(j, checked + 1)[None, :] <= n_steps[lv, None])
-                tt, jj = np.nonzero(grid)
+                last = np.where(reach_step[lv] > 0, np.minimum(reach_step[lv], checked), checked)
+                tt, jj = np.nonzero(np.arange(j, checked + 1)[None, :] <= last[:, None])
                 tt, jj = lv[tt], jj + (j - 1)
-                free_grid[tt, jj] = edges_free(P[tt, jj], P[tt, jj + 1])
+                P0 = np.stack(pos[j - 1:checked], axis=1)  # [n, checked - j + 1, dim]
+                P1 = np.stack(pos[j:checked + 1], axis=1)
+                free_grid[tt, jj] = edges_free(P0[tt, jj - (j - 1)], P1[tt, jj - (j - 1)])
             li = np.nonzero(live)[0]
             free_c = free_grid[li, j - 1]
             # The reference walks li in order: a blocked step ends its target, a free one is
