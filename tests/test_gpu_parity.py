@@ -361,12 +361,13 @@ def test_paths_valid_under_fp64_checker():
 
 @pytest.mark.parametrize("pass1_rounds,team_max,epoch_rounds,slots",
                          [(0, 1, 0, 0), (0, 2, 0, 0), (0, 8, 0, 0), (1, 4, 0, 0), (2, 32, 0, 0), (-1, 0, 0, 0),
-                          (0, 512, 1, 0), (3, 0, 2, 0), (1, 16, 7, 0), (8, 0, 0, 0), (0, 0, 3, 300)])
+                          (0, 512, 1, 0), (3, 0, 2, 0), (1, 16, 7, 0), (8, 0, 0, 0), (16, 0, 0, 0), (0, 0, 3, 300)])
 def test_team_schedules_bit_exact(pass1_rounds, team_max, epoch_rounds, slots):
     """Hand-over budget, team size and epoch length change who computes what, never the result.
     Teams are slots / open queries work-groups (102 for 5 queries on 512 slots: not a power of
-    two); a pass-1 budget above 4 rounds exercises the early hand-over once every query has
-    started; 300 slots give a grid barrier of 300 work-groups (arriving in 4 groups)."""
+    two); a pass-1 budget above 12 rounds (the pass's minimum) exercises the early hand-over once
+    every query has started; 300 slots give a grid barrier of 300 work-groups (arriving in 4
+    groups)."""
     robot = scenes.make_arm(0, 64, 200)
     qids = [0, 1, 2, 26, 455]  # 26 needs 158 rounds, 455 never connects within the cap
     ctxs, starts, goals, nodes = [], [], [], []
