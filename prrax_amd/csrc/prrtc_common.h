@@ -363,10 +363,14 @@ __device__ __forceinline__ PRRTC_LDS int* lds_i(int* p) { return (PRRTC_LDS int*
 // COH: nodes at indices >= coh_from may have been written by other work-groups since the last
 // grid-wide fence, so they are read with coherent loads; older nodes never change and are read
 // through the caches (a wave reads coherently if any of its nodes needs it).
+// Rows: the range is cut into rows of nn_row(MAXD) nodes (one scan step of the work-group), and
+// only rows row0, row0 + rows, ... are scanned, so that `rows` parts of one tree split it
+// evenly, its newest nodes (read coherently) included.
+__host__ __device__ constexpr int nn_row_nodes_per_thread(int maxd) { return maxd <= 8 ? 2 : 1; }
 template <int MAXD, bool COH = false>
 __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int end, int dim,
                         const PRRTC_LDS float* qsh, PRRTC_LDS float* red_d, PRRTC_LDS int* red_i,
-                        int coh_from = 0) {
+                        int coh_from = 0, int row0 = 0, int rows = 1) {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int nw = (nt + kWave - 1) / kWave;
     float bd[kEC];
@@ -406,12 +410,13 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
         // the end are harmless), and kept raw until then, so that no wait for them comes before
         // the scoring.  A thread visits its nodes in increasing index order (i before i + nt), so
         // on equal distances the lower index still wins.
+        const int stride = rows * 2 * nt, first = begin + row0 * 2 * nt + tid;
         float ra[MAXD], rb[MAXD];
-        if (begin + tid < end) {
-            load_raw(begin + tid, ra);
-            load_raw(begin + tid + nt, rb);
+        if (first < end) {
+            load_raw(first, ra);
+            load_raw(first + nt, rb);
         }
-        for (int i = begin + tid; i < end; i += 2 * nt) {
+        for (int i = first; i < end; i += stride) {
             asm volatile("" : "+v"(qoff));
             // (this copy also frees ra / rb for the next loads: without it -- dim folded to 7 in
             // the 7-term instantiation -- the compiler waited for the prefetch at the top of the
@@ -422,8 +427,8 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
                 xa[d] = (d < dim) ? ra[d] : 0.0f;
                 xb[d] = (d < dim) ? rb[d] : 0.0f;
             }
-            load_raw(i + 2 * nt, ra);
-            load_raw(i + 3 * nt, rb);
+            load_raw(i + stride, ra);
+            load_raw(i + stride + nt, rb);
             const bool hb = i + nt < end;
 #pragma unroll
             for (int c = 0; c < kEC; ++c) {
@@ -441,7 +446,7 @@ __device__ void nn_tile(const float* __restrict__ tree, int ncap, int begin, int
             }
         }
     } else {
-        for (int i = begin + tid; i < end; i += nt) {
+        for (int i = begin + row0 * nt + tid; i < end; i += rows * nt) {
             asm volatile("" : "+v"(qoff));
             float x[MAXD];
             load_node(i, x);

@@ -523,15 +523,14 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             const int c0 = t ? n0 + (k - nt0) * kEC : k * kEC;
             const int nc = min(kEC, (t ? n0 + n1 : n0) - c0);
             const int sz = t ? sz1 : sz0;
-            // node range [lo, hi) = [sz * part / parts, sz * (part + 1) / parts), in 32 bits when
-            // the products fit (a 64-bit division is a long instruction sequence)
-            const unsigned long long p1 = static_cast<unsigned long long>(sz) * (part + 1);
-            const unsigned long long p0 = p1 - static_cast<unsigned long long>(sz);
-            const int lo = (p1 >> 32) ? static_cast<int>(p0 / parts)
-                                      : static_cast<int>(static_cast<unsigned>(p0) / static_cast<unsigned>(parts));
-            const int hi = (p1 >> 32) ? static_cast<int>(p1 / parts)
-                                      : static_cast<int>(static_cast<unsigned>(p1) / static_cast<unsigned>(parts));
-            if (lo >= hi) continue;
+            // part `part` of `parts` scans rows part, part + parts, ... of `span` nodes (nn_tile),
+            // so the newest nodes, which are read coherently, are shared out too
+            const int span = nn_row_nodes_per_thread(MAXD) * nt;
+            const int full = sz / span, tail = sz - full * span;
+            const int mine = ((full > part) ? (full - part + parts - 1) / parts : 0) * span +
+                             ((full % parts == part) ? tail : 0);  // nodes of the part's rows
+            if (mine == 0) continue;
+            const int lo = 0, hi = sz;
             for (int q = tid; q < kEC * MAXD; q += nt) {
                 const int c = q / MAXD, d = q - (q / MAXD) * MAXD;
                 qsh[c * kMaxDim + d] = (c < nc && d < dim) ? qpt(list[c0 + c], d) : 0.0f;
@@ -543,17 +542,17 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
             if constexpr (MAXD == 8) {
                 if (dim == 7) {
                     nn_tile<7, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
-                                    t ? coh_b : coh_a);
+                                    t ? coh_b : coh_a, part, parts);
                     done7 = true;
                 }
             }
             if (!done7)
                 nn_tile<MAXD, COH>(t ? TB : TA, ncap, lo, hi, dim, lds_f(qsh), lds_f(red_d), lds_i(red_i),
-                                   t ? coh_b : coh_a);
+                                   t ? coh_b : coh_a, part, parts);
             if (tid == 0) {  // node bytes, passes and (query, node) pairs (no return value: the atomics do not wait)
-                global_add(&a.stats[16], 4ull * static_cast<unsigned long long>(dim) * static_cast<unsigned>(hi - lo));
+                global_add(&a.stats[16], 4ull * static_cast<unsigned long long>(dim) * static_cast<unsigned>(mine));
                 global_add(&a.stats[17], 1ull);
-                global_add(&a.stats[19], static_cast<unsigned long long>(nc) * static_cast<unsigned>(hi - lo));
+                global_add(&a.stats[19], static_cast<unsigned long long>(nc) * static_cast<unsigned>(mine));
             }
             if (tid < nc) gmin64(&keys[list[c0 + tid]], nn_key(red_d[tid * 16], red_i[tid * 16]));
             __syncthreads();
