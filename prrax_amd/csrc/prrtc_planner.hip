@@ -128,8 +128,9 @@ enum { SC_QUERY = 0, SC_SIZE_A, SC_SIZE_B, SC_STATUS, SC_EVENT, SC_OVERFLOW, SC_
 // Pass 1 gives a query at least this many rounds; after that, once no query is left to start,
 // the query moves on to pass 2 (teams) instead of spending the rest of its budget on one
 // work-group while the others idle.  (C3 steps: 1 / 2 / 4 / 8 / 12 / 16 / 24 rounds 259.5 /
-// 256.1 / 253.3 / 251.3 / 251.4 / 251.4 / 260.0 ms.)
-constexpr int kPass1MinRounds = 12;
+// 256.1 / 253.3 / 251.3 / 251.4 / 251.4 / 260.0 ms; 256 dual-arm queries at S = 128, all started
+// at once: 4 / 6 / 8 / 12 / never 51.0 / 51.0 / 50.7 / 59.0 / 65.4 ms.)
+constexpr int kPass1MinRounds = 8;
 
 // ---------------------------------------------------------------------------------- sampler
 // Explorer e's order of the 16 Halton bases {3, 5, ..., 59}, as indices into that list.

@@ -365,7 +365,7 @@ def test_paths_valid_under_fp64_checker():
 def test_team_schedules_bit_exact(pass1_rounds, team_max, epoch_rounds, slots):
     """Hand-over budget, team size and epoch length change who computes what, never the result.
     Teams are slots / open queries work-groups (102 for 5 queries on 512 slots: not a power of
-    two); a pass-1 budget above 12 rounds (the pass's minimum) exercises the early hand-over once
+    two); a pass-1 budget above 8 rounds (the pass's minimum) exercises the early hand-over once
     every query has started; 300 slots give a grid barrier of 300 work-groups (arriving in 4
     groups)."""
     robot = scenes.make_arm(0, 64, 200)
