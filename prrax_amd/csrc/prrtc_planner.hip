@@ -815,7 +815,8 @@ __device__ void run_query(const PlanArgs& a, const QueryView& V, char* lds, Team
                     __syncthreads();
                 }
             }
-            int us = m * nwaves + wid;
+            // consecutive units (often one explorer's) go to different members' CUs
+            int us = wid * W + m;
             int start = ((m * nwaves + wid) * S) / (W * nwaves);
             for (;;) {
                 int e = -1, jj = 0, k0 = 0;
