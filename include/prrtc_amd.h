@@ -138,7 +138,7 @@ typedef struct {
     int32_t team_max;     /* planner: largest team (work-groups per query); 0 = auto (512) */
     int32_t epoch_rounds; /* planner: most rounds a query runs between two re-formings of the teams of
                              the team pass (epochs run 4, 8, 16, ... rounds up to this); 0 = auto (64).
-                             With 2 to 32 open queries the teams are sized by each query's estimated
+                             With 2 to 64 open queries the teams are sized by each query's estimated
                              remaining rounds and an epoch lasts about 250 us per round of its budget
                              (at most 4x its rounds), so teams of different speeds end it together */
     int32_t fk_batch;     /* planner: configurations per wave in one FK + check tile (8 or 16; a check

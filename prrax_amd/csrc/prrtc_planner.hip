@@ -1216,7 +1216,9 @@ __global__ __launch_bounds__(512) void plan_pass1(PlanArgs a) {
 // to the query with the latest estimated finish (remaining rounds x round time), lowest list
 // index on ties, up to the team cap.  start[k] .. start[k + 1] are the work-groups of team k (the
 // query at position k of the list); work-groups from start[ncont] on have no team this epoch.
-constexpr int kTailMax = 32;
+// One lane per query, so at most one wave's worth (tails of up to 16 / 32 / 64 queries: C3 245.2
+// / 244.5 / 244.7 ms, 256 dual-arm queries 52.1 / 50.6 / 50.2 ms).
+constexpr int kTailMax = kWave;
 constexpr float kTeamLatency = 43.0f;
 constexpr float kGrowthExp = 1.15f;
 // Epoch time per round of budget.  C3 steps (with kGrowthExp 1.15): 150 us 255.3 ms, 200 us
