@@ -36,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes of one batched step of the 1024-query workload (plan_pass1 + plan_pass2): rocprofv3
 # FETCH_SIZE (x 2.0, calibrated on a plain 1 GiB copy) plus WRITE_SIZE (x 1.0), measured by the
 # profile runs named in "source", not by this run.  Reported only for the launch shape they ran.
-TRAFFIC = {"bytes_per_step": 25.24e9,
+TRAFFIC = {"bytes_per_step": 25.28e9,
            "source": "profiles/plan_batched_1024q_hbm_counters.md: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
-                     "of the timed step of the same workload at this launch shape (244.8 ms step)",
+                     "of the timed step of the same workload at this launch shape (244.2 ms step)",
            "shape": {"num_new_samples": 512, "slots": 256, "threads": 512, "pass1_rounds": 24, "team_max": 512,
                      "fk_batch": 16}}
 
